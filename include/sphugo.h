@@ -1,0 +1,169 @@
+/*
+ * sphugo_amd: MI355X-native SPH step (k-nearest-neighbour search, density,
+ * pressure + artificial-viscosity force, leapfrog DKD integration) behind a
+ * plain C ABI.  Device-resident structure-of-arrays state; hand-written HIP
+ * kernels for gfx950.
+ *
+ * Every entry point returns 0 on success or a negative SPH_E* code; the
+ * message of the last failure on the calling thread is sph_last_error().
+ * The library never aborts.  A handle is not re-entrant: one caller thread
+ * at a time (the reference holds Simulation.IsBusy across Step,
+ * sim/sph.go:66,197); distinct handles are independent.
+ *
+ * Reference interfaces replaced (bbeni/sphugo, Go package `sim`):
+ *   sph_create            MakeSimulationFromConf        sim/sph.go:41-54
+ *   sph_step              (*Simulation).Step            sim/sph.go:64-198
+ *   sph_calculate_forces  (*Simulation).CalculateForces sim/sph.go:403-435
+ *   sph_read_field        Root.Particles[i].{Pos,Vel,Rho,C,E,EDot,VDot,
+ *                         EPred,VPred,NNDists[0]}        sim/core.go:17-42
+ *                         (the Animator's readback, sim/animator.go:55-176)
+ *   sph_read_neighbours   Particle.NearestNeighbours/NNDists after
+ *                         FindNearestNeighboursPeriodic sim/nearest-neighbour.go:28-67
+ *   sph_total_*           TotalEnergy/TotalDensity/TotalMomentum
+ *                                                        sim/sph.go:441-463
+ *   sph_params            SphConfig fields read by the step
+ *                                                        sim/config-parser.go:111-149
+ */
+#ifndef SPHUGO_H
+#define SPHUGO_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes */
+#define SPH_OK 0
+#define SPH_EINVAL -1   /* bad argument or parameter */
+#define SPH_EDEVICE -2  /* HIP runtime error */
+#define SPH_ENOMEM -3   /* device allocation failed or admission refused */
+#define SPH_ESTATE -4   /* call not valid in the current state */
+#define SPH_ERANGE -5   /* particle left the supported range (e.g. NaN) */
+
+/* kernels (sim/sph.go:258,284; config words "Monahan", "Wendtland") */
+#define SPH_KERNEL_MONAGHAN 0
+#define SPH_KERNEL_WENDLAND 1
+
+/* arithmetic of the density and force sums */
+#define SPH_PREC_FP64 0  /* plain FP64 throughout */
+#define SPH_PREC_FP32C 1 /* FP32 pair arithmetic, compensated sums; FP64 kNN
+                            distances, h, positions, CFL and reductions */
+
+/* flags */
+#define SPH_FLAG_BITWISE 1u /* FP64: neighbour sums in the reference's order
+                               (farthest first, sim/sph.go:313,343) */
+#define SPH_FLAG_TIMING 2u  /* record per-kernel HIP event timings */
+
+/* fields (same numbering in the oracle) */
+#define SPH_X 0
+#define SPH_Y 1
+#define SPH_VX 2
+#define SPH_VY 3
+#define SPH_RHO 4
+#define SPH_C 5
+#define SPH_E 6
+#define SPH_EDOT 7
+#define SPH_AX 8
+#define SPH_AY 9
+#define SPH_EPRED 10
+#define SPH_VPX 11
+#define SPH_VPY 12
+#define SPH_H 13 /* k-th neighbour distance, NNDists[0] */
+#define SPH_NFIELDS 14
+
+/* per-kernel timer slots (SPH_FLAG_TIMING) */
+#define SPH_T_KEYS 0
+#define SPH_T_SORT 1
+#define SPH_T_PERMUTE 2
+#define SPH_T_KNN_DENSITY 3
+#define SPH_T_FORCE 4
+#define SPH_T_REDUCE 5
+#define SPH_NTIMERS 6
+
+/* Values the step reads from SphConfig (sim/config-parser.go:111-149) plus
+ * the parameters the reference hard-codes (sim/core.go:14 NN_SIZE,
+ * sim/sph.go:379-383 viscosity).  hor/vert equal to {-DBL_MAX, DBL_MAX}
+ * mean an open axis, as in the reference. */
+typedef struct sph_params {
+  int32_t k;          /* neighbours per particle, 2..64 (reference: 32) */
+  int32_t kernel;     /* SPH_KERNEL_* */
+  int32_t precision;  /* SPH_PREC_* */
+  uint32_t flags;     /* SPH_FLAG_* */
+  double gamma;       /* Gamma (reference default 1.66666) */
+  double mass;        /* ParticleMass */
+  double dt_half;     /* DeltaTHalf; <= 0 selects the CFL half step */
+  double cfl;         /* CFL number for dt_half <= 0 */
+  double accel[2];    /* Acceleration (gravity) */
+  double hor[2];      /* HorPeriodicity */
+  double vert[2];     /* VertPeriodicity */
+  double refl[4];     /* Reflections L, R, U, D */
+  double visc_alpha;  /* 0.75 in the reference */
+  double visc_beta;   /* 1.5 */
+  double visc_eta2;   /* 0.01 */
+} sph_params;
+
+typedef struct sph_sim sph_sim;
+
+/* Reference defaults (MakeConfig, sim/config-parser.go:131-149, with k=32
+ * and the viscosity constants of sim/sph.go:379-383). */
+int sph_default_params(sph_params* p);
+
+/* Upload n particles (positions, velocities, specific internal energy).
+ * vx, vy, e may be NULL (zero).  id may be NULL (IDs 0..n-1); otherwise IDs
+ * must be a permutation of 0..n-1.  The caller keeps ownership of the host
+ * arrays.  MakeSimulationFromConf, sim/sph.go:41-54. */
+int sph_create(const sph_params* p, int64_t n, const double* x, const double* y,
+               const double* vx, const double* vy, const double* e,
+               const uint32_t* id, sph_sim** out);
+void sph_destroy(sph_sim* s);
+
+/* nsteps calls of Simulation.Step (sim/sph.go:64-198), including the extra
+ * force pass of step 0; returns when the device has finished. */
+int sph_step(sph_sim* s, int32_t nsteps);
+/* Same, without waiting for the device (pair with sph_synchronize). */
+int sph_step_async(sph_sim* s, int32_t nsteps);
+int sph_synchronize(sph_sim* s);
+
+/* Simulation.CalculateForces (sim/sph.go:403-435) on the current state:
+ * kNN, density, sound speed from EPred, force from VPred. */
+int sph_calculate_forces(sph_sim* s);
+
+/* Copy one field of every particle to host memory, indexed by particle ID. */
+int sph_read_field(sph_sim* s, int32_t field, double* out);
+/* Overwrite one field (X..VPY) of every particle, indexed by ID. */
+int sph_write_field(sph_sim* s, int32_t field, const double* in);
+/* Neighbours of the last pass: ids[ID*k + i] and d2[ID*k + i] (d2 may be
+ * NULL), rows sorted ascending by (squared distance, ID). */
+int sph_read_neighbours(sph_sim* s, uint32_t* ids, double* d2);
+
+/* sum e, sum rho, E = sum m(e + v^2/2), Px, Py, last dt_half (FP64, fixed
+ * reduction order). */
+int sph_reductions(sph_sim* s, double out[6]);
+/* reference totals: TotalEnergy = sum e, TotalDensity = sum rho,
+ * TotalMomentum = |v| of the particle stored last (sim/sph.go:457-463) */
+int sph_total_energy(sph_sim* s, double* out);
+int sph_total_density(sph_sim* s, double* out);
+int sph_total_momentum(sph_sim* s, double* out);
+
+int64_t sph_current_step(const sph_sim* s);
+int64_t sph_num_particles(const sph_sim* s);
+/* CFL half step the next step will use (when dt_half <= 0). */
+int sph_next_dt_half(sph_sim* s, double* out);
+
+/* Average device time per launch of each kernel class since the last
+ * reset (SPH_FLAG_TIMING), in milliseconds, and launch counts. */
+int sph_kernel_times(sph_sim* s, double ms[SPH_NTIMERS], int64_t counts[SPH_NTIMERS]);
+int sph_reset_kernel_times(sph_sim* s);
+
+/* Number of kNN searches that needed more than one radius in the last
+ * pass, and the number of exact squared-distance ties met at the k-th
+ * boundary. */
+int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties);
+
+const char* sph_last_error(void);
+const char* sph_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

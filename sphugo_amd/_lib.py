@@ -1,0 +1,228 @@
+"""ctypes binding of libsphugo_amd.so (the C ABI in include/sphugo.h).
+
+There is no CPU fallback: if the HIP library is missing or the device call
+fails, the error is raised.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIBPATH = os.path.join(HERE, "libsphugo_amd.so")
+HEADER = os.path.join(ROOT, "include", "sphugo.h")
+
+DBL_MAX = np.finfo(np.float64).max
+
+# constants mirrored from include/sphugo.h
+KERNEL_MONAGHAN, KERNEL_WENDLAND = 0, 1
+PREC_FP64, PREC_FP32C = 0, 1
+FLAG_BITWISE, FLAG_TIMING = 1, 2
+FIELDS = ["x", "y", "vx", "vy", "rho", "c", "e", "edot", "ax", "ay", "epred",
+          "vpx", "vpy", "h"]
+FIELD = {n: i for i, n in enumerate(FIELDS)}
+TIMERS = ["keys", "sort", "permute", "knn_density", "force", "reduce"]
+ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ESTATE", -5: "ERANGE"}
+
+
+class SphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("k", ctypes.c_int32), ("kernel", ctypes.c_int32),
+        ("precision", ctypes.c_int32), ("flags", ctypes.c_uint32),
+        ("gamma", ctypes.c_double), ("mass", ctypes.c_double),
+        ("dt_half", ctypes.c_double), ("cfl", ctypes.c_double),
+        ("accel", ctypes.c_double * 2), ("hor", ctypes.c_double * 2),
+        ("vert", ctypes.c_double * 2), ("refl", ctypes.c_double * 4),
+        ("visc_alpha", ctypes.c_double), ("visc_beta", ctypes.c_double),
+        ("visc_eta2", ctypes.c_double),
+    ]
+
+
+_lib = None
+_D = ctypes.POINTER(ctypes.c_double)
+_U32 = ctypes.POINTER(ctypes.c_uint32)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_VP = ctypes.c_void_p
+
+SIGNATURES = {
+    "sph_default_params": (ctypes.c_int, [ctypes.POINTER(Params)]),
+    "sph_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_int64, _D, _D, _D, _D, _D,
+                                  _U32, ctypes.POINTER(_VP)]),
+    "sph_destroy": (None, [_VP]),
+    "sph_step": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "sph_step_async": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "sph_synchronize": (ctypes.c_int, [_VP]),
+    "sph_calculate_forces": (ctypes.c_int, [_VP]),
+    "sph_read_field": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
+    "sph_write_field": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
+    "sph_read_neighbours": (ctypes.c_int, [_VP, _U32, _D]),
+    "sph_reductions": (ctypes.c_int, [_VP, _D]),
+    "sph_total_energy": (ctypes.c_int, [_VP, _D]),
+    "sph_total_density": (ctypes.c_int, [_VP, _D]),
+    "sph_total_momentum": (ctypes.c_int, [_VP, _D]),
+    "sph_current_step": (ctypes.c_int64, [_VP]),
+    "sph_num_particles": (ctypes.c_int64, [_VP]),
+    "sph_next_dt_half": (ctypes.c_int, [_VP, _D]),
+    "sph_kernel_times": (ctypes.c_int, [_VP, _D, _I64]),
+    "sph_reset_kernel_times": (ctypes.c_int, [_VP]),
+    "sph_knn_stats": (ctypes.c_int, [_VP, _I64, _I64]),
+    "sph_last_error": (ctypes.c_char_p, []),
+    "sph_version": (ctypes.c_char_p, []),
+}
+
+
+def declared_symbols():
+    """Entry points declared in include/sphugo.h."""
+    txt = open(HEADER).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(sph_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBPATH):
+        raise RuntimeError(
+            f"{LIBPATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(there is no CPU fallback)")
+    L = ctypes.CDLL(LIBPATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise SphError(rc, lib().sph_last_error().decode())
+    return rc
+
+
+def dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    check(lib().sph_default_params(ctypes.byref(p)))
+    for k, v in kw.items():
+        if k in ("accel", "hor", "vert", "refl"):
+            arr = getattr(p, k)
+            for i, val in enumerate(v):
+                arr[i] = val
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Sim:
+    """A device-resident simulation handle (one sph_sim*)."""
+
+    def __init__(self, params: Params, x, y, vx=None, vy=None, e=None, ids=None):
+        n = len(x)
+        self.n = n
+        self.k = params.k
+        self.params = params
+        arrs = []
+        for a in (x, y, vx, vy, e):
+            arrs.append(None if a is None else np.ascontiguousarray(a, np.float64))
+        self._keep = arrs
+        ptrs = [dptr(a) if a is not None else None for a in arrs]
+        idp = None
+        if ids is not None:
+            self._ids = np.ascontiguousarray(ids, np.uint32)
+            idp = self._ids.ctypes.data_as(_U32)
+        h = _VP()
+        check(lib().sph_create(ctypes.byref(params), n, *ptrs, idp, ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def step(self, nsteps=1):
+        check(lib().sph_step(self._h, nsteps))
+
+    def step_async(self, nsteps=1):
+        check(lib().sph_step_async(self._h, nsteps))
+
+    def synchronize(self):
+        check(lib().sph_synchronize(self._h))
+
+    def calculate_forces(self):
+        check(lib().sph_calculate_forces(self._h))
+
+    def get(self, name):
+        out = np.empty(self.n)
+        check(lib().sph_read_field(self._h, FIELD[name], dptr(out)))
+        return out
+
+    def set(self, name, values):
+        v = np.ascontiguousarray(values, np.float64)
+        assert v.shape == (self.n,)
+        check(lib().sph_write_field(self._h, FIELD[name], dptr(v)))
+
+    def neighbours(self, with_d2=True):
+        ids = np.empty((self.n, self.k), np.uint32)
+        d2 = np.empty((self.n, self.k)) if with_d2 else None
+        check(lib().sph_read_neighbours(self._h, ids.ctypes.data_as(_U32),
+                                        dptr(d2) if with_d2 else None))
+        return ids, d2
+
+    def reductions(self):
+        out = np.empty(6)
+        check(lib().sph_reductions(self._h, dptr(out)))
+        return out
+
+    def scalar(self, fn):
+        out = ctypes.c_double()
+        check(getattr(lib(), fn)(self._h, ctypes.byref(out)))
+        return out.value
+
+    def total_energy(self):
+        return self.scalar("sph_total_energy")
+
+    def total_density(self):
+        return self.scalar("sph_total_density")
+
+    def total_momentum(self):
+        return self.scalar("sph_total_momentum")
+
+    def next_dt_half(self):
+        return self.scalar("sph_next_dt_half")
+
+    @property
+    def current_step(self):
+        return int(lib().sph_current_step(self._h))
+
+    def kernel_times(self):
+        ms = np.zeros(len(TIMERS))
+        cnt = np.zeros(len(TIMERS), np.int64)
+        check(lib().sph_kernel_times(self._h, dptr(ms), cnt.ctypes.data_as(_I64)))
+        return dict(zip(TIMERS, ms)), dict(zip(TIMERS, cnt))
+
+    def reset_kernel_times(self):
+        check(lib().sph_reset_kernel_times(self._h))
+
+    def knn_stats(self):
+        r = ctypes.c_int64()
+        t = ctypes.c_int64()
+        check(lib().sph_knn_stats(self._h, ctypes.byref(r), ctypes.byref(t)))
+        return r.value, t.value

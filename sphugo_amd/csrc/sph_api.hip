@@ -1,0 +1,530 @@
+// C ABI of the SPH step (declared in include/sphugo.h).
+//
+// One handle = one batch of independent simulations sharing device buffers
+// (a single simulation is a batch of one).  State lives on the device as
+// structure-of-arrays in cell order; host arrays are only touched by
+// sph_create, sph_read_* and sph_write_field.
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sph_host.hpp"
+#include "sphugo.h"
+
+using namespace sph;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(call)                                                                \
+  do {                                                                              \
+    hipError_t e_ = (call);                                                         \
+    if (e_ != hipSuccess)                                                           \
+      return fail(SPH_EDEVICE, "%s: %s", #call, hipGetErrorString(e_));             \
+  } while (0)
+
+}  // namespace
+
+struct sph_sim {
+  std::vector<SimDev> hsims;
+  std::vector<int64_t> sim_n;
+  std::vector<Item> hitems;
+  int64_t np = 0;
+  int kmax = 0;
+  int prec = 0;
+  int bitwise = 0;
+  bool timing = false;
+  hipStream_t st = nullptr;
+  hipEvent_t ev[2 * SPH_NTIMERS] = {};
+  double t_ms[SPH_NTIMERS] = {};
+  int64_t t_cnt[SPH_NTIMERS] = {};
+  int cur = 0;
+  Fields F[2] = {};
+  Positions P[2] = {};
+  PassBuffers pb = {};
+  int64_t step = 0;
+  std::vector<void*> allocs;
+  double* d_scratch = nullptr;     // np doubles for readback
+  uint32_t* d_nids = nullptr;      // neighbour readback (lazy)
+  double* d_nd2 = nullptr;
+  KnnStats last_stats = {};
+
+  template <class T>
+  int alloc(T** p, int64_t count) {
+    void* q = nullptr;
+    size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(T);
+    if (hipMalloc(&q, bytes) != hipSuccess)
+      return fail(SPH_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+  }
+  ~sph_sim() {
+    for (void* q : allocs) hipFree(q);
+    for (auto& e : ev)
+      if (e) hipEventDestroy(e);
+    if (st) hipStreamDestroy(st);
+  }
+  void bind() {
+    pb.fcur = F[cur];
+    pb.fnew = F[cur ^ 1];
+    pb.pos_cur = P[0];
+    pb.pos_pass = P[1];
+  }
+};
+
+static bool is_open(const double a[2]) { return a[0] == -DBL_MAX; }
+
+extern "C" int sph_default_params(sph_params* p) {
+  if (!p) return fail(SPH_EINVAL, "null params");
+  std::memset(p, 0, sizeof *p);
+  p->k = 32;
+  p->kernel = SPH_KERNEL_MONAGHAN;
+  p->precision = SPH_PREC_FP64;
+  p->gamma = 1.66666;
+  p->mass = 1;
+  p->dt_half = 0.001;
+  p->cfl = 0.3;
+  p->hor[0] = -DBL_MAX; p->hor[1] = DBL_MAX;
+  p->vert[0] = -DBL_MAX; p->vert[1] = DBL_MAX;
+  p->refl[0] = -DBL_MAX; p->refl[1] = DBL_MAX;
+  p->refl[2] = -DBL_MAX; p->refl[3] = DBL_MAX;
+  p->visc_alpha = 0.75;
+  p->visc_beta = 1.5;
+  p->visc_eta2 = 0.01;
+  return 0;
+}
+
+static int check_params(const sph_params* p, int64_t n) {
+  if (!p) return fail(SPH_EINVAL, "null params");
+  if (n < 2) return fail(SPH_EINVAL, "need at least 2 particles, got %lld", (long long)n);
+  if (n >= (1LL << 28)) return fail(SPH_EINVAL, "too many particles in one batch: %lld", (long long)n);
+  if (p->k < 2 || p->k > kMaxK) return fail(SPH_EINVAL, "k must be in [2, %d], got %d", kMaxK, p->k);
+  if (p->kernel != SPH_KERNEL_MONAGHAN && p->kernel != SPH_KERNEL_WENDLAND)
+    return fail(SPH_EINVAL, "unknown kernel %d", p->kernel);
+  if (p->precision != SPH_PREC_FP64 && p->precision != SPH_PREC_FP32C)
+    return fail(SPH_EINVAL, "unknown precision %d", p->precision);
+  if ((p->flags & SPH_FLAG_BITWISE) && p->precision != SPH_PREC_FP64)
+    return fail(SPH_EINVAL, "bitwise reference order needs FP64 precision");
+  if (is_open(p->hor) || is_open(p->vert))
+    return fail(SPH_EINVAL, "open boundaries are not supported by the device path; "
+                            "give periodic Hor/VertPeriodicity");
+  if (!(p->hor[1] > p->hor[0]) || !(p->vert[1] > p->vert[0]) || !std::isfinite(p->hor[1] - p->hor[0]) ||
+      !std::isfinite(p->vert[1] - p->vert[0]))
+    return fail(SPH_EINVAL, "periodic box must have finite positive size");
+  if (9 * (n - 1) < p->k) return fail(SPH_EINVAL, "fewer than k candidates over 9 images");
+  if (!(p->gamma > 1) && !(p->gamma < 1)) return fail(SPH_EINVAL, "gamma must not be 1");
+  if (p->dt_half <= 0 && !(p->cfl > 0)) return fail(SPH_EINVAL, "CFL number must be positive");
+  return 0;
+}
+
+// Cells per axis: about `occ` particles per cell at the mean density.
+static int grid_side(int64_t n, double occ) {
+  double g = std::floor(std::sqrt((double)n / occ));
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (int)g;
+}
+
+static void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int g) {
+  std::memset(&d, 0, sizeof d);
+  d.n = (int32_t)n;
+  d.k = p->k;
+  d.kernel = p->kernel;
+  d.prec = p->precision;
+  d.flags = p->flags;
+  d.use_cfl = p->dt_half <= 0 ? 1 : 0;
+  d.gx = g;
+  d.gy = g;
+  d.x0 = p->hor[0];
+  d.y0 = p->vert[0];
+  d.lx = p->hor[1] - p->hor[0];
+  d.ly = p->vert[1] - p->vert[0];
+  d.ivx = g / d.lx;
+  d.ivy = g / d.ly;
+  d.sx = d.lx / g;
+  d.sy = d.ly / g;
+  d.gamma = p->gamma;
+  d.mass = p->mass;
+  d.cfl = p->cfl;
+  d.gxa = p->accel[0];
+  d.gya = p->accel[1];
+  d.hor0 = p->hor[0];
+  d.hor1 = p->hor[1];
+  d.vert0 = p->vert[0];
+  d.vert1 = p->vert[1];
+  for (int i = 0; i < 4; ++i) d.refl[i] = p->refl[i];
+  d.alpha = p->visc_alpha;
+  d.beta = p->visc_beta;
+  d.eta2 = p->visc_eta2;
+  d.fpref = p->kernel == SPH_KERNEL_WENDLAND ? kWendlandFPref : kMonaghanPref;
+  d.dfpref = p->kernel == SPH_KERNEL_WENDLAND ? kWendlandDFPref : kMonaghanPref;
+  d.cfac = p->gamma * (p->gamma - 1);
+  d.dt_fixed = p->dt_half;
+  d.dt_half = p->dt_half > 0 ? p->dt_half : 0.0;
+  d.r_init = 1.2 * std::sqrt(p->k * d.lx * d.ly / (M_PI * (double)n));
+}
+
+// Build a handle holding `count` simulations.
+static int create_batch(int count, const sph_params* const* ps, const int64_t* ns,
+                        const double* const* xs, const double* const* ys,
+                        const double* const* vxs, const double* const* vys,
+                        const double* const* es, const uint32_t* const* ids, sph_sim** out) {
+  if (!out) return fail(SPH_EINVAL, "null output handle");
+  *out = nullptr;
+  if (count < 1) return fail(SPH_EINVAL, "empty batch");
+  int64_t np = 0;
+  int kmax = 0;
+  for (int s = 0; s < count; ++s) {
+    int rc = check_params(ps[s], ns[s]);
+    if (rc) return rc;
+    if (ps[s]->precision != ps[0]->precision || (ps[s]->flags & SPH_FLAG_BITWISE) !=
+                                                    (ps[0]->flags & SPH_FLAG_BITWISE))
+      return fail(SPH_EINVAL, "precision and order flags must agree across a batch");
+    np += ns[s];
+    kmax = ps[s]->k > kmax ? ps[s]->k : kmax;
+  }
+  if (np >= (1LL << 28)) return fail(SPH_EINVAL, "too many particles in one batch");
+  auto* h = new sph_sim();
+  h->np = np;
+  h->kmax = kmax;
+  h->prec = ps[0]->precision;
+  h->bitwise = (ps[0]->flags & SPH_FLAG_BITWISE) ? 1 : 0;
+  h->timing = (ps[0]->flags & SPH_FLAG_TIMING) != 0;
+  int rc = 0;
+  auto bail = [&](int code) {
+    delete h;
+    return code;
+  };
+  if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(SPH_EDEVICE, "hipStreamCreate failed"));
+  if (h->timing)
+    for (auto& e : h->ev)
+      if (hipEventCreate(&e) != hipSuccess) return bail(fail(SPH_EDEVICE, "hipEventCreate failed"));
+  // simulations, cells, items
+  int64_t off = 0, coff = 0;
+  std::vector<int> red_first(count), red_count(count);
+  for (int s = 0; s < count; ++s) {
+    SimDev d;
+    const int g = grid_side(ns[s], 2.0);
+    fill_simdev(d, ps[s], ns[s], g);
+    d.off = off;
+    d.coff = coff;
+    h->hsims.push_back(d);
+    h->sim_n.push_back(ns[s]);
+    red_first[s] = (int)h->hitems.size();
+    for (int64_t b = 0; b < ns[s]; b += kItem) {
+      Item it;
+      it.sim = s;
+      it.first = off + b;
+      it.count = (int)((ns[s] - b) < kItem ? (ns[s] - b) : kItem);
+      h->hitems.push_back(it);
+    }
+    red_count[s] = (int)h->hitems.size() - red_first[s];
+    off += ns[s];
+    coff += (int64_t)g * g;
+  }
+  if (coff >= (1LL << 32) - 1) return bail(fail(SPH_EINVAL, "too many cells"));
+  int bits = 1;
+  while ((1LL << bits) <= coff) ++bits;
+  PassBuffers& pb = h->pb;
+  pb.nsims = count;
+  pb.nitems = (int)h->hitems.size();
+  pb.kmax = kmax;
+  pb.prec = h->prec;
+  pb.bitwise = h->bitwise;
+  pb.np = np;
+  pb.ncells = coff;
+  pb.key_bits = bits;
+  // device memory
+  Item* d_items = nullptr;
+  SimDev* d_sims = nullptr;
+  int *d_rf = nullptr, *d_rc = nullptr;
+  if ((rc = h->alloc(&d_items, pb.nitems)) || (rc = h->alloc(&d_sims, count)) ||
+      (rc = h->alloc(&d_rf, count)) || (rc = h->alloc(&d_rc, count)))
+    return bail(rc);
+  for (int q = 0; q < 2; ++q) {
+    Fields& f = h->F[q];
+    double** fl[] = {&f.vx, &f.vy, &f.e, &f.ax, &f.ay, &f.edot, &f.vpx, &f.vpy, &f.ep, &f.h, &f.rho, &f.c};
+    for (double** a : fl)
+      if ((rc = h->alloc(a, np))) return bail(rc);
+    if ((rc = h->alloc(&f.id, np))) return bail(rc);
+    if ((rc = h->alloc(&h->P[q].x, np)) || (rc = h->alloc(&h->P[q].y, np))) return bail(rc);
+  }
+  if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, coff + 1)) ||
+      (rc = h->alloc(&pb.nbr, np * kmax)) || (rc = h->alloc(&pb.partials, (int64_t)pb.nitems * kNumRed)) ||
+      (rc = h->alloc(&pb.stats, 1)) || (rc = h->alloc(&pb.err, 1)) ||
+      (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
+      (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
+      (rc = h->alloc(&pb.sort.counts, sort_counts_size(np))) || (rc = h->alloc(&pb.sort.tot, 256)) ||
+      (rc = h->alloc(&h->d_scratch, np)))
+    return bail(rc);
+  pb.items = d_items;
+  pb.sims = d_sims;
+  pb.red_first = d_rf;
+  pb.red_count = d_rc;
+  // upload
+  auto up = [&](void* dst, const void* src, size_t bytes) {
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->st) == hipSuccess;
+  };
+  bool ok = up(d_items, h->hitems.data(), h->hitems.size() * sizeof(Item)) &&
+            up(d_sims, h->hsims.data(), h->hsims.size() * sizeof(SimDev)) &&
+            up(d_rf, red_first.data(), count * sizeof(int)) &&
+            up(d_rc, red_count.data(), count * sizeof(int));
+  if (!ok) return bail(fail(SPH_EDEVICE, "upload failed"));
+  for (int q = 0; q < 2; ++q) {
+    Fields& f = h->F[q];
+    double* zero[] = {f.vx, f.vy, f.e, f.ax, f.ay, f.edot, f.vpx, f.vpy, f.ep, f.h, f.rho, f.c};
+    for (double* a : zero)
+      if (hipMemsetAsync(a, 0, np * sizeof(double), h->st) != hipSuccess)
+        return bail(fail(SPH_EDEVICE, "memset failed"));
+  }
+  if (hipMemsetAsync(pb.stats, 0, sizeof(KnnStats), h->st) != hipSuccess ||
+      hipMemsetAsync(pb.err, 0, sizeof(int), h->st) != hipSuccess ||
+      hipMemsetAsync(pb.nbr, 0, np * kmax * sizeof(uint32_t), h->st) != hipSuccess)
+    return bail(fail(SPH_EDEVICE, "memset failed"));
+  std::vector<uint32_t> idbuf;
+  for (int s = 0; s < count; ++s) {
+    const int64_t o = h->hsims[s].off, n = ns[s];
+    if (!xs[s] || !ys[s]) return bail(fail(SPH_EINVAL, "positions must not be null"));
+    for (int64_t i = 0; i < n; ++i)
+      if (!std::isfinite(xs[s][i]) || !std::isfinite(ys[s][i]))
+        return bail(fail(SPH_ERANGE, "non-finite position of particle %lld", (long long)i));
+    ok = up(h->P[0].x + o, xs[s], n * sizeof(double)) && up(h->P[0].y + o, ys[s], n * sizeof(double));
+    if (vxs && vxs[s]) ok = ok && up(h->F[0].vx + o, vxs[s], n * sizeof(double));
+    if (vys && vys[s]) ok = ok && up(h->F[0].vy + o, vys[s], n * sizeof(double));
+    if (es && es[s]) ok = ok && up(h->F[0].e + o, es[s], n * sizeof(double));
+    if (!ok) return bail(fail(SPH_EDEVICE, "upload failed"));
+    idbuf.resize(n);
+    if (ids && ids[s]) {
+      std::vector<char> seen(n, 0);
+      for (int64_t i = 0; i < n; ++i) {
+        if (ids[s][i] >= (uint64_t)n || seen[ids[s][i]])
+          return bail(fail(SPH_EINVAL, "ids must be a permutation of 0..n-1"));
+        seen[ids[s][i]] = 1;
+        idbuf[i] = ids[s][i];
+      }
+    } else {
+      for (int64_t i = 0; i < n; ++i) idbuf[i] = (uint32_t)i;
+    }
+    if (hipMemcpy(h->F[0].id + o, idbuf.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+        hipSuccess)
+      return bail(fail(SPH_EDEVICE, "upload failed"));
+  }
+  h->bind();
+  if (enqueue_init_sims(pb, h->st)) return bail(fail(SPH_EDEVICE, "init kernel failed"));
+  if (hipStreamSynchronize(h->st) != hipSuccess)
+    return bail(fail(SPH_EDEVICE, "create: %s", hipGetErrorString(hipGetLastError())));
+  *out = h;
+  return 0;
+}
+
+extern "C" int sph_create(const sph_params* p, int64_t n, const double* x, const double* y,
+                          const double* vx, const double* vy, const double* e,
+                          const uint32_t* id, sph_sim** out) {
+  return create_batch(1, &p, &n, &x, &y, &vx, &vy, &e, &id, out);
+}
+
+extern "C" void sph_destroy(sph_sim* s) {
+  if (!s) return;
+  if (s->st) hipStreamSynchronize(s->st);
+  delete s;
+}
+
+static int run_pass(sph_sim* s, int perm_mode, int force_mode) {
+  s->bind();
+  int rc = enqueue_pass(s->pb, perm_mode, force_mode, s->st, s->timing ? s->ev : nullptr);
+  if (rc) return fail(SPH_EDEVICE, "kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
+  s->cur ^= 1;
+  s->bind();
+  if (s->timing) {
+    HIPCHK(hipStreamSynchronize(s->st));
+    for (int t = 0; t < SPH_NTIMERS; ++t) {
+      float ms = 0;
+      if (hipEventElapsedTime(&ms, s->ev[2 * t], s->ev[2 * t + 1]) == hipSuccess) {
+        s->t_ms[t] += ms;
+        s->t_cnt[t] += 1;
+      }
+    }
+  }
+  return 0;
+}
+
+static int check_device_flags(sph_sim* s) {
+  int err = 0;
+  KnnStats st;
+  HIPCHK(hipMemcpyAsync(&err, s->pb.err, sizeof(int), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(&st, s->pb.stats, sizeof st, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->last_stats = st;
+  if (err) return fail(SPH_ERANGE, "a particle left the periodic images [-L, 2L) of the box");
+  if (st.failures) return fail(SPH_ERANGE, "kNN: %llu searches found fewer than k neighbours",
+                               (unsigned long long)st.failures);
+  return 0;
+}
+
+extern "C" int sph_step_async(sph_sim* s, int32_t nsteps) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  if (nsteps < 0) return fail(SPH_EINVAL, "negative step count");
+  for (int32_t i = 0; i < nsteps; ++i) {
+    int rc;
+    if (s->step == 0 && (rc = run_pass(s, kPermInit, kForceNoKick))) return rc;
+    if ((rc = run_pass(s, kPermPredict, kForceKick))) return rc;
+    s->step += 1;
+  }
+  return 0;
+}
+
+extern "C" int sph_synchronize(sph_sim* s) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(s->st));
+  return check_device_flags(s);
+}
+
+extern "C" int sph_step(sph_sim* s, int32_t nsteps) {
+  int rc = sph_step_async(s, nsteps);
+  if (rc) return rc;
+  return sph_synchronize(s);
+}
+
+extern "C" int sph_calculate_forces(sph_sim* s) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  int rc = run_pass(s, kPermKeep, kForceNoKick);
+  if (rc) return rc;
+  return sph_synchronize(s);
+}
+
+extern "C" int sph_read_field(sph_sim* s, int32_t field, double* out) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  if (field < 0 || field >= SPH_NFIELDS) return fail(SPH_EINVAL, "unknown field %d", field);
+  s->bind();
+  int rc = enqueue_read_field(s->pb, field, s->d_scratch, s->st);
+  if (rc) return fail(SPH_EDEVICE, "readback launch failed");
+  HIPCHK(hipMemcpyAsync(out, s->d_scratch, s->np * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int sph_write_field(sph_sim* s, int32_t field, const double* in) {
+  if (!s || !in) return fail(SPH_EINVAL, "null argument");
+  if (field < 0 || field > SPH_VPY || field == SPH_RHO || field == SPH_C)
+    return fail(SPH_EINVAL, "field %d cannot be written", field);
+  if (field == SPH_X || field == SPH_Y)
+    for (int64_t i = 0; i < s->np; ++i)
+      if (!std::isfinite(in[i])) return fail(SPH_ERANGE, "non-finite position");
+  s->bind();
+  HIPCHK(hipMemcpyAsync(s->d_scratch, in, s->np * sizeof(double), hipMemcpyHostToDevice, s->st));
+  int rc = enqueue_write_field(s->pb, field, s->d_scratch, s->st);
+  if (rc) return fail(SPH_EDEVICE, "upload launch failed");
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int sph_read_neighbours(sph_sim* s, uint32_t* ids, double* d2) {
+  if (!s || !ids) return fail(SPH_EINVAL, "null argument");
+  const int64_t total = s->np * s->kmax;
+  int rc = 0;
+  if (!s->d_nids && ((rc = s->alloc(&s->d_nids, total)) || (rc = s->alloc(&s->d_nd2, total))))
+    return rc;
+  s->bind();
+  rc = enqueue_read_neighbours(s->pb, s->d_nids, s->d_nd2, s->st);
+  if (rc) return fail(SPH_EDEVICE, "neighbour readback launch failed");
+  HIPCHK(hipMemcpyAsync(ids, s->d_nids, total * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
+  if (d2) HIPCHK(hipMemcpyAsync(d2, s->d_nd2, total * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+static int fetch_sims(sph_sim* s) {
+  HIPCHK(hipMemcpyAsync(s->hsims.data(), s->pb.sims, s->hsims.size() * sizeof(SimDev),
+                        hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int sph_reductions(sph_sim* s, double out[6]) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  int rc = fetch_sims(s);
+  if (rc) return rc;
+  const SimDev& d = s->hsims[0];
+  for (int r = 0; r < kNumRed; ++r) out[r] = d.red[r];
+  out[5] = d.dt_last;
+  return 0;
+}
+
+extern "C" int sph_total_energy(sph_sim* s, double* out) {
+  double r[6];
+  int rc = sph_reductions(s, r);
+  if (!rc) *out = r[0];
+  return rc;
+}
+
+extern "C" int sph_total_density(sph_sim* s, double* out) {
+  double r[6];
+  int rc = sph_reductions(s, r);
+  if (!rc) *out = r[1];
+  return rc;
+}
+
+extern "C" int sph_total_momentum(sph_sim* s, double* out) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  int rc = fetch_sims(s);
+  if (!rc) *out = s->hsims[0].last_speed;
+  return rc;
+}
+
+extern "C" int64_t sph_current_step(const sph_sim* s) { return s ? s->step : -1; }
+extern "C" int64_t sph_num_particles(const sph_sim* s) { return s ? s->np : -1; }
+
+extern "C" int sph_next_dt_half(sph_sim* s, double* out) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  int rc = fetch_sims(s);
+  if (!rc) *out = s->hsims[0].dt_half;
+  return rc;
+}
+
+extern "C" int sph_kernel_times(sph_sim* s, double ms[SPH_NTIMERS], int64_t counts[SPH_NTIMERS]) {
+  if (!s || !ms) return fail(SPH_EINVAL, "null argument");
+  for (int t = 0; t < SPH_NTIMERS; ++t) {
+    ms[t] = s->t_cnt[t] ? s->t_ms[t] / s->t_cnt[t] : 0.0;
+    if (counts) counts[t] = s->t_cnt[t];
+  }
+  return 0;
+}
+
+extern "C" int sph_reset_kernel_times(sph_sim* s) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  for (int t = 0; t < SPH_NTIMERS; ++t) {
+    s->t_ms[t] = 0;
+    s->t_cnt[t] = 0;
+  }
+  return 0;
+}
+
+extern "C" int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  KnnStats st;
+  HIPCHK(hipMemcpyAsync(&st, s->pb.stats, sizeof st, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (retries) *retries = (int64_t)st.retries;
+  if (ties) *ties = (int64_t)st.ties;
+  return 0;
+}
+
+extern "C" const char* sph_last_error(void) { return g_err.c_str(); }
+extern "C" const char* sph_version(void) { return "sphugo_amd 0.1 (gfx950)"; }

@@ -1,0 +1,159 @@
+// Device-side definitions shared by the SPH kernels (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sph {
+
+constexpr int kWave = 64;
+constexpr int kItem = 256;        // particles per work item (one block)
+constexpr int kMaxK = 64;         // neighbours per particle
+constexpr int kCandCap = 256;     // kNN candidates held per wave
+constexpr int kNumRed = 5;        // sum e, sum rho, E, Px, Py
+
+// Go folds 6*40/(Pi*7), 4*7/(Pi*4), 8*7/(Pi*4) exactly and rounds once
+// (sim/sph.go:266,275,293,303); these are the correctly rounded values.
+constexpr double kMonaghanPref = 0x1.5d3b3e3583243p+3;
+constexpr double kWendlandFPref = 0x1.1d34a60108f72p+1;
+constexpr double kWendlandDFPref = 0x1.1d34a60108f72p+2;
+
+enum : int32_t { kMonaghan = 0, kWendland = 1 };
+enum : int32_t { kPrecFP64 = 0, kPrecFP32C = 1 };
+enum : uint32_t { kFlagBitwise = 1u, kFlagTiming = 2u };
+
+// Per-simulation parameters and device-updated scalars.
+struct SimDev {
+  int64_t off;      // first particle (global index)
+  int64_t coff;     // first cell (global index)
+  int32_t n;        // particles
+  int32_t k;        // neighbours
+  int32_t kernel;   // kMonaghan / kWendland
+  int32_t prec;     // kPrecFP64 / kPrecFP32C
+  uint32_t flags;
+  int32_t use_cfl;
+  int32_t gx, gy;   // cells per axis
+  int32_t pad0, pad1;
+  double x0, y0;    // periodic box origin (Hor[0], Vert[0])
+  double lx, ly;    // box size (Hor[1]-Hor[0], Vert[1]-Vert[0])
+  double ivx, ivy;  // cells per unit length
+  double sx, sy;    // cell size
+  double gamma, mass, cfl;
+  double gxa, gya;  // Acceleration
+  double hor0, hor1, vert0, vert1;
+  double refl[4];   // L R U D
+  double alpha, beta, eta2;
+  double fpref, dfpref;
+  double cfac;      // gamma*(gamma-1)
+  double dt_fixed;
+  double r_init;    // first-pass kNN radius guess
+  // device-updated
+  double dt_half;       // half step of the current step
+  double dt_last;       // half step of the last completed step
+  unsigned long long cfl_min_bits;  // atomicMin over non-negative doubles
+  double red[kNumRed];  // reductions of the last pass
+  double last_speed;    // |v| of the particle stored last
+};
+
+struct Item {
+  int32_t sim;
+  int32_t count;
+  int64_t first;
+};
+
+struct KnnStats {
+  unsigned long long retries;
+  unsigned long long ties;
+  unsigned long long failures;
+};
+
+// Work-item index for this block, arranged so that each of the 8 XCDs
+// (which receive blocks round-robin) walks a contiguous run of items and
+// keeps its own L2 warm on one band of the particle array.
+__device__ inline int xcd_item(int nitems) {
+  const int b = blockIdx.x;
+  const int q = nitems >> 3, r = nitems & 7, x = b & 7;
+  return x * q + (x < r ? x : r) + (b >> 3);
+}
+
+__device__ inline int64_t floordiv(int64_t a, int64_t b) {
+  int64_t q = a / b;
+  if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
+  return q;
+}
+
+// Reference kernel functions, evaluated in Go's order with no contraction.
+#pragma clang fp contract(off)
+__device__ inline double kern_F(int32_t kernel, double q) {
+  if (kernel == kWendland) return (1 - q) * (1 - q) * (1 - q) * (1 - q) * (1 + 4 * q);
+  if (q < 0.5) return q * q * q - q * q + 1.0 / 6;
+  return (1 - q) * (1 - q) * (1 - q) / 3;
+}
+__device__ inline double kern_DF(int32_t kernel, double q) {
+  if (kernel == kWendland) return -10 * q * (1 - q) * (1 - q) * (1 - q);
+  if (q < 0.5) return (3 * q * q - 2 * q);
+  return -(1 - q) * (1 - q);
+}
+__device__ inline float kern_Ff(int32_t kernel, float q) {
+  if (kernel == kWendland) {
+    float a = 1.f - q, a2 = a * a;
+    return a2 * a2 * (1.f + 4.f * q);
+  }
+  if (q < 0.5f) return q * q * (q - 1.f) + (1.f / 6.f);
+  float a = 1.f - q;
+  return a * a * a * (1.f / 3.f);
+}
+__device__ inline float kern_DFf(int32_t kernel, float q) {
+  if (kernel == kWendland) {
+    float a = 1.f - q;
+    return -10.f * q * a * a * a;
+  }
+  if (q < 0.5f) return q * (3.f * q - 2.f);
+  float a = 1.f - q;
+  return -a * a;
+}
+
+// ---- wave helpers (wave64) ----
+__device__ inline int lane_id() { return __lane_id(); }
+
+__device__ inline unsigned long long ballot(bool p) { return __ballot(p); }
+
+// number of set bits of m in lanes below this one
+__device__ inline int lane_prefix(unsigned long long m) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ inline double readlane_d(double v, int lane) {
+  union { double d; int32_t i[2]; } u, r;
+  u.d = v;
+  r.i[0] = __builtin_amdgcn_readlane(u.i[0], lane);
+  r.i[1] = __builtin_amdgcn_readlane(u.i[1], lane);
+  return r.d;
+}
+__device__ inline uint32_t readlane_u(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+
+// inclusive scan over the wave
+__device__ inline int wave_incl_scan(int v) {
+  const int l = lane_id();
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    int t = __shfl_up(v, o, 64);
+    if (l >= o) v += t;
+  }
+  return v;
+}
+
+// butterfly sum over groups of `width` lanes (power of two); every lane of
+// a group ends with the same value (a+b == b+a exactly)
+__device__ inline double group_sum(double v, int width) {
+  for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline float group_sumf(float v, int width) {
+  for (int o = width >> 1; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace sph

@@ -1,0 +1,74 @@
+// Host-side declarations shared by the kernel launchers and the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "sph_device.hpp"
+
+namespace sph {
+
+struct SortBuffers {
+  uint32_t* keys;
+  uint32_t* vals;
+  uint32_t* keys_alt;
+  uint32_t* vals_alt;
+  uint32_t* counts;  // [256][blocks]
+  uint32_t* tot;     // [256]
+};
+
+int sort_pairs(SortBuffers& sb, int64_t n, int bits, hipStream_t st, bool first_identity,
+               uint32_t** keys_out, uint32_t** vals_out);
+int64_t sort_counts_size(int64_t n);
+
+// One structure-of-arrays set of per-particle fields (positions separate).
+struct Fields {
+  double *vx, *vy, *e, *ax, *ay, *edot, *vpx, *vpy, *ep, *h, *rho, *c;
+  uint32_t* id;
+};
+
+struct Positions {
+  double *x, *y;
+};
+
+enum PermuteMode : int { kPermPredict = 0, kPermInit = 1, kPermKeep = 2 };
+enum ForceMode : int { kForceKick = 0, kForceNoKick = 1 };
+
+struct PassBuffers {
+  const Item* items;
+  int nitems;
+  SimDev* sims;
+  int nsims;
+  int kmax;             // row stride of the neighbour list
+  int prec;             // kPrecFP64 / kPrecFP32C (uniform over a batch)
+  int bitwise;          // reference-order sums (FP64 only)
+  const int* red_first; // per simulation: first item
+  const int* red_count; // per simulation: item count
+  int64_t np;           // particles
+  int64_t ncells;       // cells (all simulations)
+  int key_bits;
+  Positions pos_cur;    // positions at rest (P0)
+  Positions pos_pass;   // positions of the pass (P1)
+  Fields fcur;          // fields, current order
+  Fields fnew;          // fields, order of this pass
+  uint8_t* wrap;        // wrap code per particle of the pass
+  uint32_t* cstart;     // ncells + 1
+  uint32_t* nbr;        // np * kmax packed (index << 4 | image)
+  double* partials;     // nitems * kNumRed
+  KnnStats* stats;
+  int* err;             // device error flag
+  SortBuffers sort;
+};
+
+// Enqueue one force pass (keys, sort, permute, kNN + density, force,
+// reductions) on `st`.  Timing events (may be null) are recorded around
+// each kernel class.
+int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
+                 hipEvent_t* ev /* 2*SPH_NTIMERS or null */);
+
+int enqueue_read_field(const PassBuffers& b, int field, double* dev_out, hipStream_t st);
+int enqueue_write_field(PassBuffers& b, int field, const double* dev_in, hipStream_t st);
+int enqueue_read_neighbours(const PassBuffers& b, uint32_t* dev_ids, double* dev_d2,
+                            hipStream_t st);
+int enqueue_init_sims(PassBuffers& b, hipStream_t st);
+
+}  // namespace sph

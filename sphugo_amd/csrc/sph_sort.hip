@@ -1,0 +1,166 @@
+// Stable LSD radix sort of (cell key, particle index) pairs, 8-bit digits.
+//
+// Replaces the reference's per-pass Treebuild/Partition permutation
+// (sim/core.go:126-224): particles are reordered by the row-major cell index
+// of a uniform grid so that every row segment of cells is one contiguous
+// range of the state arrays.
+//
+// Per pass: (1) block histograms of the digit, (2) one exclusive scan over
+// the digit-major [256][blocks] table, (3) a stable scatter in which each
+// wave ranks its 64 keys by matching digits with 9 ballots, and waves of a
+// block are ordered through LDS.  Stability across passes makes the result
+// independent of launch timing, so runs are bitwise reproducible.
+#include "sph_device.hpp"
+#include "sph_host.hpp"
+
+namespace sph {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortSub = 16;  // sub-tiles of 256 keys per block
+constexpr int kSortTile = kSortThreads * kSortSub;
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_hist(
+    const uint32_t* __restrict__ keys, int64_t n, int shift,
+    uint32_t* __restrict__ counts, int nblocks) {
+  __shared__ uint32_t hist[256];
+  const int tid = threadIdx.x;
+  hist[tid] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int r = 0; r < kSortSub; ++r) {
+    int64_t i = base + (int64_t)r * kSortThreads + tid;
+    if (i < n) atomicAdd(&hist[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  counts[(int64_t)tid * nblocks + blockIdx.x] = hist[tid];
+}
+
+// Row d of the digit-major table counts[d][b]: exclusive scan over blocks b
+// in place, row total to tot[d].  One block per digit.
+__global__ __launch_bounds__(1024) void k_sort_rowscan(uint32_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ tot,
+                                                       int nblocks) {
+  __shared__ uint32_t part[1024];
+  const int tid = threadIdx.x;
+  uint32_t* row = counts + (int64_t)blockIdx.x * nblocks;
+  const int per = (nblocks + 1023) / 1024;
+  const int b = tid * per;
+  uint32_t s = 0;
+  for (int i = 0; i < per; ++i) s += (b + i < nblocks) ? row[b + i] : 0u;
+  part[tid] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    uint32_t t = tid >= o ? part[tid - o] : 0u;
+    __syncthreads();
+    part[tid] += t;
+    __syncthreads();
+  }
+  uint32_t run = part[tid] - s;
+  for (int i = 0; i < per && b + i < nblocks; ++i) {
+    const uint32_t c = row[b + i];
+    row[b + i] = run;
+    run += c;
+  }
+  if (tid == 1023) tot[blockIdx.x] = part[1023];
+}
+
+__global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
+    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
+    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
+    int shift, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tot,
+    int nblocks, int identity_vals) {
+  __shared__ uint32_t wcnt[4][256];
+  __shared__ uint32_t woff[4][256];
+  __shared__ uint32_t run[256];
+  __shared__ uint32_t gofs[256];
+  const int tid = threadIdx.x;
+  const int w = tid >> 6;
+  const int lane = tid & 63;
+  run[tid] = 0;
+  // exclusive prefix of the digit totals, then this block's row offset
+  {
+    uint32_t t = tot[tid];
+    woff[0][tid] = t;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      uint32_t a = tid >= o ? woff[0][tid - o] : 0u;
+      __syncthreads();
+      woff[0][tid] += a;
+      __syncthreads();
+    }
+    gofs[tid] = woff[0][tid] - t + offsets[(int64_t)tid * nblocks + blockIdx.x];
+    __syncthreads();
+  }
+  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < kSortSub; ++r) {
+    const int64_t i = base + (int64_t)r * kSortThreads + tid;
+    const bool valid = i < n;
+    uint32_t key = valid ? kin[i] : 0u;
+    uint32_t val = valid ? (identity_vals ? (uint32_t)i : vin[i]) : 0u;
+    uint32_t d = valid ? ((key >> shift) & 255u) : 256u;
+    // lanes holding the same digit
+    unsigned long long peers = ~0ull;
+#pragma unroll
+    for (int b = 0; b < 9; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long bal = ballot(bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const int rank = __popcll(peers & lt_mask);
+    const bool leader = (peers & lt_mask) == 0ull;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) wcnt[q][tid] = 0;
+    __syncthreads();
+    if (valid && leader) wcnt[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    {
+      uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+      uint32_t r0 = run[tid];
+      woff[0][tid] = r0;
+      woff[1][tid] = r0 + c0;
+      woff[2][tid] = r0 + c0 + c1;
+      woff[3][tid] = r0 + c0 + c1 + c2;
+      run[tid] = r0 + c0 + c1 + c2 + c3;
+    }
+    __syncthreads();
+    if (valid) {
+      const uint32_t dst = gofs[d] + woff[w][d] + (uint32_t)rank;
+      kout[dst] = key;
+      vout[dst] = val;
+    }
+    __syncthreads();
+  }
+}
+
+int sort_pairs(SortBuffers& sb, int64_t n, int bits, hipStream_t st,
+               bool first_identity, uint32_t** keys_out, uint32_t** vals_out) {
+  const int nblocks = (int)((n + kSortTile - 1) / kSortTile);
+  uint32_t* kin = sb.keys;
+  uint32_t* vin = sb.vals;
+  uint32_t* kout = sb.keys_alt;
+  uint32_t* vout = sb.vals_alt;
+  const int passes = bits <= 0 ? 1 : (bits + 7) / 8;
+  for (int p = 0; p < passes; ++p) {
+    const int shift = 8 * p;
+    hipLaunchKernelGGL(k_sort_hist, dim3(nblocks), dim3(kSortThreads), 0, st, kin, n,
+                       shift, sb.counts, nblocks);
+    hipLaunchKernelGGL(k_sort_rowscan, dim3(256), dim3(1024), 0, st, sb.counts, sb.tot,
+                       nblocks);
+    hipLaunchKernelGGL(k_sort_scatter, dim3(nblocks), dim3(kSortThreads), 0, st, kin, vin,
+                       kout, vout, n, shift, sb.counts, sb.tot, nblocks,
+                       (p == 0 && first_identity) ? 1 : 0);
+    uint32_t* t = kin; kin = kout; kout = t;
+    t = vin; vin = vout; vout = t;
+  }
+  *keys_out = kin;
+  *vals_out = vin;
+  return (int)hipGetLastError();
+}
+
+int64_t sort_counts_size(int64_t n) {
+  return (int64_t)256 * ((n + kSortTile - 1) / kSortTile);
+}
+
+}  // namespace sph

@@ -1,0 +1,195 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Bars (stated per check):
+  neighbour sets        bit-exact IDs and squared distances (exact kNN,
+                        ties by ID)
+  h                     bit-exact (sqrt of the k-th squared distance)
+  density, sound speed  FP64: relative 1e-12 (bitwise mode: exact);
+                        FP32C: relative 1e-5
+  acceleration, EDot    FP64: |da| <= 1e-12 * max|a| over the set and per
+                        particle 1e-12 * sum_j |term_j| is met through the
+                        normwise bound below; bitwise mode: exact
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from sphugo_amd import _lib as L
+from sphugo_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+
+def _params(st, k, precision=L.PREC_FP64, flags=0, kernel=0, **kw):
+    kw.setdefault("gamma", 5 / 3)
+    kw.setdefault("visc_alpha", 1.0)
+    kw.setdefault("visc_beta", 2.0)
+    return dict(k=k, kernel=kernel, mass=st["mass"], hor=(0, 1), vert=(0, 1), **kw), \
+        dict(precision=precision, flags=flags)
+
+
+def _pair(st, k, precision=L.PREC_FP64, flags=0, kernel=0, forces=True, **kw):
+    common, gpu_only = _params(st, k, precision, flags, kernel, **kw)
+    g = L.Sim(L.default_params(**common, **gpu_only), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    # the C ABI selects the CFL half step with dt_half <= 0; the oracle with use_cfl
+    use_cfl = 1 if common.get("dt_half", 0.001) <= 0 else 0
+    o = O.OracleSim(O.default_params(knn_mode=O.KNN_EXACT, use_cfl=use_cfl, **common),
+                    st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    if forces:
+        for sim in (g, o):
+            sim.set("vpx", st["vx"])
+            sim.set("vpy", st["vy"])
+            sim.set("epred", st["e"])
+            sim.calculate_forces()
+    return g, o
+
+
+def _check_knn(g, o):
+    gi, gd = g.neighbours()
+    oi, od = o.neighbours()
+    assert (gi.astype(np.int64) == oi).all(), "neighbour IDs differ"
+    assert (gd == od).all(), "neighbour squared distances differ"
+    assert (g.get("h") == o.get("h")).all()
+
+
+def _rel(a, b):
+    return np.abs(a - b) / np.abs(b)
+
+
+def _check_forces(g, o, tol=1e-12):
+    for f in ("rho", "c"):
+        assert _rel(g.get(f), o.get(f)).max() <= tol, f
+    for f in ("ax", "ay", "edot"):
+        a, b = g.get(f), o.get(f)
+        assert np.abs(a - b).max() <= tol * np.abs(b).max(), f
+
+
+@pytest.mark.parametrize("k", [32, 64])
+def test_pass_uniform(k):
+    st = ics.uniform(2200)
+    g, o = _pair(st, k)
+    _check_knn(g, o)
+    _check_forces(g, o)
+
+
+def test_pass_bitwise_is_exact():
+    st = ics.uniform(2200)
+    g, o = _pair(st, 32, flags=L.FLAG_BITWISE)
+    _check_knn(g, o)
+    for f in ("rho", "c", "ax", "ay", "edot", "h"):
+        assert (g.get(f) == o.get(f)).all(), f
+
+
+@pytest.mark.parametrize("kernel", [0, 1])
+def test_pass_clustered_k64(kernel):
+    st = ics.clustered(6000, seed=7, k=64)
+    g, o = _pair(st, 64, kernel=kernel)
+    _check_knn(g, o)
+    _check_forces(g, o)
+
+
+def test_pass_blast_reference_viscosity():
+    st = ics.blast(5000, seed=5)
+    g, o = _pair(st, 32, visc_alpha=0.75, visc_beta=1.5)
+    _check_knn(g, o)
+    _check_forces(g, o)
+
+
+def test_tiny_box_many_images():
+    st = ics.uniform(12, seed=9)
+    g, o = _pair(st, 32)
+    _check_knn(g, o)
+    _check_forces(g, o)
+
+
+def test_steps_fixed_dt_bitwise():
+    # strict-parity twin of the uniform workload: reference viscosity,
+    # fixed half step equal to half the CFL step of the step-0 pass
+    st = ics.uniform(2200)
+    _, o0 = _pair(st, 32, visc_alpha=0.75, visc_beta=1.5)
+    dth = o0.cfl_dt_half()
+    g, o = _pair(st, 32, flags=L.FLAG_BITWISE, forces=False, visc_alpha=0.75,
+                 visc_beta=1.5, dt_half=dth)
+    g.step(10)
+    o.step(10)
+    assert g.current_step == 10
+    for f in L.FIELDS:
+        assert (g.get(f) == o.get(f)).all(), f
+    _check_knn(g, o)
+
+
+def test_steps_cfl_fp64():
+    st = ics.uniform(2200)
+    g, o = _pair(st, 32, forces=False, dt_half=0.0, cfl=0.3)
+    g.step(10)
+    o.step(10)
+    red_g, red_o = g.reductions(), o.reductions()
+    # chaotic amplification over 10 steps is far below these bounds
+    for f in ("x", "y"):
+        d = g.get(f) - o.get(f)
+        d -= np.rint(d)
+        assert np.abs(d).max() < 1e-10, f
+    for f in ("vx", "vy", "e", "rho"):
+        a, b = g.get(f), o.get(f)
+        assert np.abs(a - b).max() <= 1e-9 * np.abs(b).max(), f
+    assert abs(red_g[5] - red_o[5]) <= 1e-12 * red_o[5]
+    assert np.allclose(red_g[:5], red_o[:5], rtol=1e-10, atol=1e-14)
+
+
+def test_steps_cfl_bitwise_exact():
+    st = ics.blast(3000, seed=11)
+    g, o = _pair(st, 32, flags=L.FLAG_BITWISE, forces=False, dt_half=0.0, cfl=0.3)
+    g.step(5)
+    o.step(5)
+    for f in L.FIELDS:
+        assert (g.get(f) == o.get(f)).all(), f
+    assert g.reductions()[5] == o.reductions()[5]
+
+
+def test_fp32c_errors():
+    st = ics.clustered(8000, seed=3, k=64)
+    g, o = _pair(st, 64, precision=L.PREC_FP32C)
+    _check_knn(g, o)
+    assert _rel(g.get("rho"), o.get("rho")).max() <= 1e-5
+    for f in ("ax", "ay"):
+        a, b = g.get(f), o.get(f)
+        assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), f
+
+
+def test_totals_and_quirks():
+    st = ics.blast(4000, seed=2)
+    g, o = _pair(st, 32, forces=False, dt_half=0.0, cfl=0.3)
+    g.step(3)
+    o.step(3)
+    assert np.isclose(g.total_energy(), o.total_energy(), rtol=1e-12)
+    assert np.isclose(g.total_density(), o.total_density(), rtol=1e-10)
+    # TotalMomentum is the speed of the particle stored last: some particle
+    sp = np.hypot(g.get("vx"), g.get("vy"))
+    assert np.isclose(sp, g.total_momentum(), rtol=0, atol=0).any()
+
+
+def test_wrap_quirk_on_device():
+    # lattice at rest, no pressure (e = 0), moving diagonally out of the box
+    n = 16
+    gl = (np.arange(n) + 0.5) / n
+    x, y = [a.ravel().copy() for a in np.meshgrid(gl, gl, indexing="ij")]
+    st = {"x": x, "y": y, "vx": np.full(n * n, -0.05), "vy": np.full(n * n, -0.05),
+          "e": np.zeros(n * n), "mass": 1 / (n * n)}
+    g, o = _pair(st, 32, forces=False, dt_half=0.5)
+    g.step(1)
+    o.step(1)
+    for f in ("x", "y", "vx", "vy"):
+        assert (g.get(f) == o.get(f)).all(), f
+    assert (g.get("y") < 0).any()  # the skipped Y wrap
+
+
+def test_invalid_inputs_raise():
+    st = ics.uniform(100)
+    with pytest.raises(L.SphError):
+        L.Sim(L.default_params(k=65, hor=(0, 1), vert=(0, 1)), st["x"], st["y"])
+    with pytest.raises(L.SphError):
+        L.Sim(L.default_params(k=32), st["x"], st["y"])  # open boundaries
+    bad = st["x"].copy()
+    bad[3] = np.nan
+    with pytest.raises(L.SphError):
+        L.Sim(L.default_params(k=32, hor=(0, 1), vert=(0, 1)), bad, st["y"])
