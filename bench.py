@@ -1,0 +1,188 @@
+"""Throughput of the SPH step on one MI355X.
+
+Workload (default): the clustered 4M case -- N = 4,194,304 particles in the
+periodic unit box, 80% in 16 Gaussians (centres uniform, sigma 0.02) and 20%
+uniform, v ~ N(0, 0.1^2) per axis, e = 1, m = 1/N, gamma = 5/3, k = 64,
+Monaghan kernel, viscosity alpha = 1, beta = 2, eta^2 = 0.01, CFL 0.3.
+
+A step is one Simulation.Step (drift, kNN, density, force, kick, drift,
+wrap), state resident in HBM.  `value` is particle-steps/s of the
+FP32-compensated form; the plain FP64 form is timed beside it.  The dominant
+kernel's roofline uses per-launch HIP event timings taken on the library's
+own stream.  The CPU baseline is the oracle's faithful restatement of the
+reference (tree build + 9-image tree kNN), timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from sphugo_amd import _lib as L  # noqa: E402
+from sphugo_amd import ics  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+METRIC = "particle-steps/s at 4M (FP32-comp vs plain FP64); p50/p99 job latency in serve"
+
+
+def algo_bytes(kernel: str, k: int) -> int:
+    """Algorithmic HBM bytes per particle of one launch (state arrays 8 B,
+    IDs and cell keys 4 B, neighbour entries 4 B; neighbour gathers count
+    as on-chip reuse)."""
+    return {
+        "keys": 32 + 4,                    # x, y, vx, vy -> key
+        "permute": 4 + 4 + 9 * 8 + 4 + 10 * 8 + 4 + 1,  # perm, key, gather, write
+        "knn_density": 16 + 1 + 8 + 8 + 4 * k + 3 * 8,  # pos, wrap, h_prev, e_pred -> list, h, rho, c
+        "force": 4 * k + 10 * 8 + 8 * 8,   # list, own state -> a, edot, v, e, x, y
+    }[kernel]
+
+
+def make_state(args):
+    if args.workload == "clustered":
+        return ics.clustered(args.n or 4194304, seed=3, k=64)
+    if args.workload == "blast":
+        return ics.blast(args.n or 262144, seed=2, k=32)
+    if args.workload == "uniform":
+        return ics.uniform(args.n or 2200, seed=12345678, k=32)
+    raise SystemExit(f"unknown workload {args.workload}")
+
+
+def params_for(st, precision):
+    return L.default_params(k=st["k"], precision=precision, mass=st["mass"], gamma=5 / 3,
+                            dt_half=0.0, cfl=0.3, hor=(0, 1), vert=(0, 1),
+                            visc_alpha=1.0, visc_beta=2.0, visc_eta2=0.01)
+
+
+def run_variant(st, precision, steps, warmup, timing_steps=3):
+    sim = L.Sim(params_for(st, precision), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    sim.step(warmup)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.step(steps)  # returns after the device finished
+    t1 = time.perf_counter()
+    dt = t1 - t0
+    # per-kernel device times (HIP events on the library's stream)
+    sim.set_timing(True)
+    sim.reset_kernel_times()
+    sim.step(timing_steps)
+    ms, cnt = sim.kernel_times()
+    sim.set_timing(False)
+    retries, ties = sim.knn_stats()
+    counters = sim.knn_counters()
+    red = sim.reductions()
+    sim.close()
+    return {
+        "seconds": dt,
+        "ms_per_step": 1e3 * dt / steps,
+        "value": st["n"] * steps / dt,
+        "kernel_ms": {k: round(v, 4) for k, v in ms.items()},
+        "knn_retries": retries,
+        "knn_counters": counters,
+        "dt_half_last": red[5],
+    }
+
+
+def cpu_baseline(threads_all: int):
+    import oracle as O
+    O.build()
+    st = ics.clustered(32768, seed=3, k=64)
+    out = {}
+    for threads in (1, threads_all):
+        p = O.default_params(k=64, knn_mode=O.KNN_TREE, gamma=5 / 3, mass=st["mass"],
+                             hor=(0, 1), vert=(0, 1), use_cfl=1, cfl=0.3,
+                             visc_alpha=1.0, visc_beta=2.0)
+        s = O.OracleSim(p, st["x"], st["y"], st["vx"], st["vy"], st["e"], threads=threads)
+        s.step(1)  # step 0 pass + first step, untimed
+        nsteps = 4
+        t0 = time.perf_counter()
+        s.step(nsteps)
+        dt = time.perf_counter() - t0
+        out[threads] = st["n"] * nsteps / dt
+    return {
+        "value": out[1], "unit": "particle-steps/s", "cores": 1, "kind": "port",
+        "sample": "clustered layout at N=32768, k=64: oracle restatement of the reference "
+                  "(Treebuild + BoundingSpheres + 9-image tree kNN + density + force + DKD), "
+                  "4 steps after step 0, 1 thread, gcc -O2 -ffp-contract=off",
+        "all_cores": {"value": out[threads_all], "cores": threads_all,
+                      "note": "OpenMP over particles in kNN/density/force; tree build serial"},
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="clustered")
+    ap.add_argument("--n", type=int, default=0)
+    ap.add_argument("--precision", default="both", choices=["both", "fp32c", "fp64"])
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
+                          "(multi-GPU sharding is a design, not a run)"}))
+        sys.exit(2)
+    st = make_state(args)
+    k = st["k"]
+    res = {}
+    if args.precision in ("both", "fp32c"):
+        res["fp32c"] = run_variant(st, L.PREC_FP32C, args.steps, args.warmup)
+        print(f"# fp32c: {res['fp32c']}", file=sys.stderr, flush=True)
+    if args.precision in ("both", "fp64"):
+        res["fp64"] = run_variant(st, L.PREC_FP64, args.steps, args.warmup)
+        print(f"# fp64: {res['fp64']}", file=sys.stderr, flush=True)
+    head = res.get("fp32c") or res["fp64"]
+    # dominant kernel and its roofline
+    km = head["kernel_ms"]
+    dom = max(("knn_density", "force", "permute", "keys"), key=lambda n: km[n])
+    bytes_launch = algo_bytes(dom, k) * st["n"]
+    ach = bytes_launch / (km[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+            "algo_bytes_per_particle": algo_bytes(dom, k), "avg_ms": km[dom]}
+    df = ("knn_density", "force")
+    t_df = sum(km[n] for n in df)
+    b_df = sum(algo_bytes(n, k) for n in df) * st["n"]
+    roof["density_force_frac"] = round(b_df / (t_df * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+    line = {
+        "metric": METRIC,
+        "value": round(head["value"], 1),
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if "fp32c" in res else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{st['name']} (k={k}, CFL 0.3, alpha=1, beta=2, periodic unit box)",
+                   "n_particles": st["n"], "k": k,
+                   "precision": "fp32c" if "fp32c" in res else "fp64"},
+        "roofline": roof,
+    }
+    if "fp64" in res:
+        line["fp64"] = {"value": round(res["fp64"]["value"], 1),
+                        "ms_per_step": round(res["fp64"]["ms_per_step"], 4),
+                        "kernel_ms": res["fp64"]["kernel_ms"]}
+    if "fp32c" in res:
+        line["fp32c_kernel_ms"] = res["fp32c"]["kernel_ms"]
+    if "fp32c" in res and "fp64" in res:
+        line["speedup_fp32c_over_fp64"] = round(res["fp32c"]["value"] / res["fp64"]["value"], 3)
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        line["cpu_baseline"] = cpu_baseline(threads)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -154,11 +154,19 @@ int sph_next_dt_half(sph_sim* s, double* out);
  * reset (SPH_FLAG_TIMING), in milliseconds, and launch counts. */
 int sph_kernel_times(sph_sim* s, double ms[SPH_NTIMERS], int64_t counts[SPH_NTIMERS]);
 int sph_reset_kernel_times(sph_sim* s);
+/* Switch per-kernel timing on (1) or off (0) for later steps.  Timing
+ * waits for the device after each pass, so whole-step throughput is
+ * measured with it off. */
+int sph_set_timing(sph_sim* s, int32_t on);
 
-/* Number of kNN searches that needed more than one radius in the last
- * pass, and the number of exact squared-distance ties met at the k-th
- * boundary. */
+/* Number of kNN searches that needed more than one radius, and the
+ * number of exact squared-distance ties met at the k-th boundary, summed
+ * over all passes since sph_create. */
 int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties);
+/* All kNN counters since sph_create: retries, ties, failures, bracket
+ * misses (k inside the inner radius, fewer than k inside the outer radius,
+ * annulus over capacity), tiles sent to the wavefront search. */
+int sph_knn_counters(sph_sim* s, int64_t out[7]);
 
 const char* sph_last_error(void);
 const char* sph_version(void);

@@ -76,6 +76,8 @@ SIGNATURES = {
     "sph_kernel_times": (ctypes.c_int, [_VP, _D, _I64]),
     "sph_reset_kernel_times": (ctypes.c_int, [_VP]),
     "sph_knn_stats": (ctypes.c_int, [_VP, _I64, _I64]),
+    "sph_set_timing": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "sph_knn_counters": (ctypes.c_int, [_VP, _I64]),
     "sph_last_error": (ctypes.c_char_p, []),
     "sph_version": (ctypes.c_char_p, []),
 }
@@ -220,6 +222,15 @@ class Sim:
 
     def reset_kernel_times(self):
         check(lib().sph_reset_kernel_times(self._h))
+
+    def set_timing(self, on):
+        check(lib().sph_set_timing(self._h, 1 if on else 0))
+
+    def knn_counters(self):
+        out = np.zeros(7, np.int64)
+        check(lib().sph_knn_counters(self._h, out.ctypes.data_as(_I64)))
+        return dict(zip(["retries", "ties", "failures", "miss_lo", "miss_hi", "miss_ann",
+                         "split_fallback"], out.tolist()))
 
     def knn_stats(self):
         r = ctypes.c_int64()

@@ -84,7 +84,6 @@ struct sph_sim {
     pb.fcur = F[cur];
     pb.fnew = F[cur ^ 1];
     pb.pos_cur = P[0];
-    pb.pos_pass = P[1];
   }
 };
 
@@ -133,15 +132,18 @@ static int check_params(const sph_params* p, int64_t n) {
   return 0;
 }
 
-// Cells per axis: about `occ` particles per cell at the mean density.
-static int grid_side(int64_t n, double occ) {
-  double g = std::floor(std::sqrt((double)n / occ));
-  if (g < 1) g = 1;
-  if (g > 4096) g = 4096;
-  return (int)g;
+// log2 of cells per axis: a power of two with about `cpp` cells per
+// particle at the mean density (Morton-ordered cells; coarser levels of the
+// same grid serve the denser regions' neighbours and the sparser regions'
+// search boxes).
+static int grid_log2(int64_t n, double cpp) {
+  int lg = 0;
+  while (lg < 13 && (double)(1LL << (2 * lg)) < cpp * (double)n) ++lg;
+  return lg;
 }
 
-static void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int g) {
+static void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
+  const int g = 1 << lg;
   std::memset(&d, 0, sizeof d);
   d.n = (int32_t)n;
   d.k = p->k;
@@ -151,6 +153,7 @@ static void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int g) {
   d.use_cfl = p->dt_half <= 0 ? 1 : 0;
   d.gx = g;
   d.gy = g;
+  d.lg = lg;
   d.x0 = p->hor[0];
   d.y0 = p->vert[0];
   d.lx = p->hor[1] - p->hor[0];
@@ -221,8 +224,9 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   std::vector<int> red_first(count), red_count(count);
   for (int s = 0; s < count; ++s) {
     SimDev d;
-    const int g = grid_side(ns[s], 2.0);
-    fill_simdev(d, ps[s], ns[s], g);
+    const int lg = grid_log2(ns[s], 4.0);
+    const int g = 1 << lg;
+    fill_simdev(d, ps[s], ns[s], lg);
     d.off = off;
     d.coff = coff;
     h->hsims.push_back(d);
@@ -245,7 +249,8 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   PassBuffers& pb = h->pb;
   pb.nsims = count;
   pb.nitems = (int)h->hitems.size();
-  pb.kmax = kmax;
+  pb.kmax = (kmax + 3) & ~3;
+  pb.kout = kmax;
   pb.prec = h->prec;
   pb.bitwise = h->bitwise;
   pb.np = np;
@@ -264,11 +269,15 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
     for (double** a : fl)
       if ((rc = h->alloc(a, np))) return bail(rc);
     if ((rc = h->alloc(&f.id, np))) return bail(rc);
-    if ((rc = h->alloc(&h->P[q].x, np)) || (rc = h->alloc(&h->P[q].y, np))) return bail(rc);
   }
+  if ((rc = h->alloc(&h->P[0].x, np)) || (rc = h->alloc(&h->P[0].y, np)) ||
+      (rc = h->alloc(&pb.xyp, np)))
+    return bail(rc);
   if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, coff + 1)) ||
-      (rc = h->alloc(&pb.nbr, np * kmax)) || (rc = h->alloc(&pb.partials, (int64_t)pb.nitems * kNumRed)) ||
+      (rc = h->alloc(&pb.nbr, np * pb.kmax)) ||
+      (rc = h->alloc((char**)&pb.rec, np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) || (rc = h->alloc(&pb.partials, (int64_t)pb.nitems * kNumRed)) ||
       (rc = h->alloc(&pb.stats, 1)) || (rc = h->alloc(&pb.err, 1)) ||
+      (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
       (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
       (rc = h->alloc(&pb.sort.counts, sort_counts_size(np))) || (rc = h->alloc(&pb.sort.tot, 256)) ||
@@ -296,7 +305,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   }
   if (hipMemsetAsync(pb.stats, 0, sizeof(KnnStats), h->st) != hipSuccess ||
       hipMemsetAsync(pb.err, 0, sizeof(int), h->st) != hipSuccess ||
-      hipMemsetAsync(pb.nbr, 0, np * kmax * sizeof(uint32_t), h->st) != hipSuccess)
+      hipMemsetAsync(pb.nbr, 0, np * pb.kmax * sizeof(uint32_t), h->st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "memset failed"));
   std::vector<uint32_t> idbuf;
   for (int s = 0; s < count; ++s) {
@@ -351,6 +360,7 @@ static int run_pass(sph_sim* s, int perm_mode, int force_mode) {
   int rc = enqueue_pass(s->pb, perm_mode, force_mode, s->st, s->timing ? s->ev : nullptr);
   if (rc) return fail(SPH_EDEVICE, "kernel launch failed: %s", hipGetErrorString((hipError_t)rc));
   s->cur ^= 1;
+  s->pb.have_h = 1;
   s->bind();
   if (s->timing) {
     HIPCHK(hipStreamSynchronize(s->st));
@@ -437,7 +447,7 @@ extern "C" int sph_write_field(sph_sim* s, int32_t field, const double* in) {
 
 extern "C" int sph_read_neighbours(sph_sim* s, uint32_t* ids, double* d2) {
   if (!s || !ids) return fail(SPH_EINVAL, "null argument");
-  const int64_t total = s->np * s->kmax;
+  const int64_t total = s->np * s->pb.kout;
   int rc = 0;
   if (!s->d_nids && ((rc = s->alloc(&s->d_nids, total)) || (rc = s->alloc(&s->d_nd2, total))))
     return rc;
@@ -516,6 +526,15 @@ extern "C" int sph_reset_kernel_times(sph_sim* s) {
   return 0;
 }
 
+extern "C" int sph_set_timing(sph_sim* s, int32_t on) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  if (on && !s->ev[0])
+    for (auto& e : s->ev)
+      if (hipEventCreate(&e) != hipSuccess) return fail(SPH_EDEVICE, "hipEventCreate failed");
+  s->timing = on != 0;
+  return 0;
+}
+
 extern "C" int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties) {
   if (!s) return fail(SPH_EINVAL, "null handle");
   KnnStats st;
@@ -523,6 +542,17 @@ extern "C" int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties) {
   HIPCHK(hipStreamSynchronize(s->st));
   if (retries) *retries = (int64_t)st.retries;
   if (ties) *ties = (int64_t)st.ties;
+  return 0;
+}
+
+extern "C" int sph_knn_counters(sph_sim* s, int64_t out[7]) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  KnnStats st;
+  HIPCHK(hipMemcpyAsync(&st, s->pb.stats, sizeof st, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  const unsigned long long v[7] = {st.retries, st.ties, st.failures, st.miss_lo,
+                                   st.miss_hi, st.miss_ann, st.split_fallback};
+  for (int i = 0; i < 7; ++i) out[i] = (int64_t)v[i];
   return 0;
 }
 

@@ -31,8 +31,9 @@ struct SimDev {
   int32_t prec;     // kPrecFP64 / kPrecFP32C
   uint32_t flags;
   int32_t use_cfl;
-  int32_t gx, gy;   // cells per axis
-  int32_t pad0, pad1;
+  int32_t gx, gy;   // cells per axis (gx = gy = 2^lg)
+  int32_t lg;       // log2 of cells per axis (Morton order)
+  int32_t pad1;
   double x0, y0;    // periodic box origin (Hor[0], Vert[0])
   double lx, ly;    // box size (Hor[1]-Hor[0], Vert[1]-Vert[0])
   double ivx, ivy;  // cells per unit length
@@ -54,6 +55,44 @@ struct SimDev {
   double last_speed;    // |v| of the particle stored last
 };
 
+// Morton (Z-order) code of a cell: bits of x at even, y at odd positions.
+__device__ inline uint32_t spread_bits(uint32_t v) {
+  v &= 0xFFFFu;
+  v = (v | (v << 8)) & 0x00FF00FFu;
+  v = (v | (v << 4)) & 0x0F0F0F0Fu;
+  v = (v | (v << 2)) & 0x33333333u;
+  v = (v | (v << 1)) & 0x55555555u;
+  return v;
+}
+__device__ inline uint32_t morton2(uint32_t x, uint32_t y) {
+  return spread_bits(x) | (spread_bits(y) << 1);
+}
+
+// Image code of a neighbour entry: the query offset o = (ix*Lx, iy*Ly) of
+// the reference's d2 = ((p_i + o) - p_j)^2, ix, iy in {-1, 0, 1}.
+__device__ inline uint32_t img_code(int ix, int iy) {
+  return (uint32_t)((ix + 1) | ((iy + 1) << 2));
+}
+__device__ inline int img_ix(uint32_t code) { return (int)(code & 3u) - 1; }
+__device__ inline int img_iy(uint32_t code) { return (int)((code >> 2) & 3u) - 1; }
+// order of images used by the oracle for (d2, ID, image) ties
+__device__ inline uint32_t img_order(uint32_t code) {
+  return (uint32_t)((img_ix(code) + 1) * 3 + (img_iy(code) + 1));
+}
+
+// Per-particle record gathered by the force kernel (written by the kNN
+// kernel once density, sound speed and h are known).
+struct alignas(16) RecD {
+  double x, y, vpx, vpy, rho, c, h, B;  // B = c^2 / (gamma rho)
+};
+struct alignas(16) RecF {
+  double x, y;
+  float vpx, vpy, rho, c;
+  float h, B, pad0, pad1;
+};
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 struct Item {
   int32_t sim;
   int32_t count;
@@ -61,9 +100,13 @@ struct Item {
 };
 
 struct KnnStats {
-  unsigned long long retries;
-  unsigned long long ties;
-  unsigned long long failures;
+  unsigned long long retries;         // searches needing a second radius
+  unsigned long long ties;            // exact d2 ties at the k-th boundary
+  unsigned long long failures;        // searches that found fewer than k
+  unsigned long long miss_lo;         // bracket: k or more inside h(1-delta)
+  unsigned long long miss_hi;         // bracket: fewer than k inside h(1+delta)
+  unsigned long long miss_ann;        // bracket: annulus over capacity
+  unsigned long long split_fallback;  // tile too wide even for 8 lanes
 };
 
 // Work-item index for this block, arranged so that each of the 8 XCDs

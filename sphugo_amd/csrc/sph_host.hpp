@@ -38,7 +38,8 @@ struct PassBuffers {
   int nitems;
   SimDev* sims;
   int nsims;
-  int kmax;             // row stride of the neighbour list
+  int kmax;             // row stride of the neighbour list (multiple of 4)
+  int kout;             // row stride of neighbour readback (largest k)
   int prec;             // kPrecFP64 / kPrecFP32C (uniform over a batch)
   int bitwise;          // reference-order sums (FP64 only)
   const int* red_first; // per simulation: first item
@@ -47,14 +48,18 @@ struct PassBuffers {
   int64_t ncells;       // cells (all simulations)
   int key_bits;
   Positions pos_cur;    // positions at rest (P0)
-  Positions pos_pass;   // positions of the pass (P1)
+  double2* xyp;         // positions of the pass, interleaved (x, y)
   Fields fcur;          // fields, current order
   Fields fnew;          // fields, order of this pass
   uint8_t* wrap;        // wrap code per particle of the pass
   uint32_t* cstart;     // ncells + 1
-  uint32_t* nbr;        // np * kmax packed (index << 4 | image)
+  uint32_t* nbr;        // np * kmax packed (index << 4 | image code)
+  void* rec;            // np force records (RecD or RecF)
   double* partials;     // nitems * kNumRed
   KnnStats* stats;
+  uint2* retry;           // kNN retry list (np entries)
+  uint32_t* retry_count;
+  int have_h;             // every particle has an h from an earlier pass
   int* err;             // device error flag
   SortBuffers sort;
 };

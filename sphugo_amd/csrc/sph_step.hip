@@ -46,7 +46,9 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
                                                 const double* __restrict__ Y,
                                                 const double* __restrict__ VX,
                                                 const double* __restrict__ VY, int mode,
-                                                uint32_t* __restrict__ keys, int* err) {
+                                                uint32_t* __restrict__ keys, int* err,
+                                                uint32_t* __restrict__ retry_count) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *retry_count = 0u;
   const Item it = items[xcd_item(nitems)];
   if ((int)threadIdx.x >= it.count) return;
   const SimDev* S = sims + it.sim;
@@ -61,13 +63,13 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
   if (!c.ok) atomicOr(err, 1);
   const int64_t cx = c.ux - floordiv(c.ux, S->gx) * S->gx;
   const int64_t cy = c.uy - floordiv(c.uy, S->gy) * S->gy;
-  keys[t] = (uint32_t)(S->coff + cy * S->gx + cx);
+  keys[t] = (uint32_t)(S->coff + morton2((uint32_t)cx, (uint32_t)cy));
 }
 
 __global__ __launch_bounds__(kItem) void k_permute(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys, int mode,
-    Positions pin, Fields fin, Positions pout, Fields fout, uint8_t* __restrict__ wrap,
+    Positions pin, Fields fin, double2* __restrict__ xyp, Fields fout, uint8_t* __restrict__ wrap,
     uint32_t* __restrict__ cstart, int64_t np, int64_t ncells) {
   const Item it = items[xcd_item(nitems)];
   if ((int)threadIdx.x >= it.count) return;
@@ -90,8 +92,7 @@ __global__ __launch_bounds__(kItem) void k_permute(
   } else {
     vpx = fin.vpx[i]; vpy = fin.vpy[i]; ep = fin.ep[i];
   }
-  pout.x[t] = x;
-  pout.y[t] = y;
+  xyp[t] = make_double2(x, y);
   fout.vx[t] = vx;
   fout.vy[t] = vy;
   fout.e[t] = e;
@@ -100,9 +101,6 @@ __global__ __launch_bounds__(kItem) void k_permute(
   fout.ep[t] = ep;
   fout.h[t] = fin.h[i];
   fout.id[t] = fin.id[i];
-  fout.ax[t] = fin.ax[i];
-  fout.ay[t] = fin.ay[i];
-  fout.edot[t] = fin.edot[i];
   CellIdx c = cell_of(S, x, y);
   const int64_t wx = floordiv(c.ux, S->gx), wy = floordiv(c.uy, S->gy);
   wrap[t] = (uint8_t)((wx + 2) | ((wy + 2) << 3));
@@ -115,375 +113,804 @@ __global__ __launch_bounds__(kItem) void k_permute(
 }
 
 // ------------------------------------------------------------------------
-// kNN: one wavefront per query.
+// kNN.
 //
-// Search: all cells meeting the disk of radius R around the query, over
-// the periodic images the reference visits (query offsets {-L,0,L}^2,
-// sim/nearest-neighbour.go:57-61).  Rows of cells are contiguous particle
-// ranges; each lane owns one row, splits it at the period boundaries into
-// pieces, and the wave walks the concatenated candidates 64 at a time.
-// Candidates with d2 <= R^2 are appended to a per-wave LDS list.  When at
-// least k arrive the k smallest (d2, ID, image) are exact: every candidate
-// not seen lies farther than R.  Otherwise R grows (or shrinks on overflow)
-// and the search repeats.  The k-th key is found by a ballot quickselect.
-constexpr int kMaxPieces = 5 * 64;
+// Candidates are the particles of Morton cells at a coarser level of the
+// grid: a coarse cell (X, Y) at level c is one contiguous range of the
+// sorted arrays, [cstart[m << 2(lg-c)], cstart[(m+1) << 2(lg-c)]) with m
+// its Morton code.  Periodic images follow the reference: a candidate j is
+// seen through the query offsets o in {-L, 0, L}^2 and its squared distance
+// is ((p_i + o) - p_j)^2 (sim/nearest-neighbour.go:57-61,72,76), self
+// excluded at every image (:79).  The k nearest are ordered by (d2, ID,
+// image) so ties resolve the same way everywhere.
 
-__device__ inline double pow2d(double x) { return x * x; }
+__device__ inline int level_for(const SimDev* S, double w, double hgt, double ncells) {
+  int c = S->lg;
+  while (c > 0 && ((double)(1 << c) * w > ncells * S->lx || (double)(1 << c) * hgt > ncells * S->ly))
+    --c;
+  return c;
+}
 
-struct KnnShared {
+__device__ inline void cell_range(const SimDev* S, const uint32_t* __restrict__ cstart, int c,
+                                  uint32_t xw, uint32_t yw, uint32_t& a, uint32_t& b) {
+  const int sh = 2 * (S->lg - c);
+  const uint64_t m = (uint64_t)morton2(xw, yw) << sh;
+  a = cstart[S->coff + m];
+  b = cstart[S->coff + m + (1ull << sh)];
+}
+
+constexpr int kV1Cells = 128;  // coarse cells of one search (<= 9 x 9 used)
+constexpr int kTileCells = 256;  // coarse cells of one tile box (<= 15 x 15 used)
+constexpr int kAnn = 12;       // annulus entries kept per query
+constexpr double kDelta = 0.02;  // bracket half-width around the last h
+constexpr int kSplit = 24;       // max tile candidates per query before a split
+
+// Per-wave scratch of the exact wavefront search.
+struct SearchShared {
   double d2[kCandCap];
   uint32_t pk[kCandCap];
-  uint32_t pst[kMaxPieces];
-  int ppref[kMaxPieces + 1];
-  int8_t ptx[kMaxPieces];
-  int8_t pty[kMaxPieces];
-  double fterm[kMaxK];
-  double sd2[kMaxK];   // reference-order staging (bitwise mode)
+  uint32_t pst[kV1Cells];
+  int ppref[kV1Cells + 1];
+  int8_t ptx[kV1Cells];
+  int8_t pty[kV1Cells];
+  double sd2[kMaxK];
   uint32_t spk[kMaxK];
+  double fterm[kMaxK];
 };
 
+// Per-wave scratch of the tile search (lane = query).
+struct TileShared {
+  uint32_t pst[kTileCells];
+  int ppref[kTileCells + 1];
+  int8_t ptx[kTileCells];
+  int8_t pty[kTileCells];
+  double2 sxy[64];
+  uint32_t spk[64];
+  uint32_t ann_pk[kAnn][64];
+};
+
+union WaveShared {
+  SearchShared s;
+  TileShared t;
+};
+
+struct SearchOut {
+  double tau;   // k-th smallest squared distance
+  double facc;  // sum of kernel values F(d/h) over the k neighbours
+};
+
+// Candidate of a piece: returns false if the reference does not visit
+// this (particle, image) pair.
+__device__ inline bool cand_image(int ptx, int pty, uint8_t wc, int& ix, int& iy) {
+  const int nx = ptx - ((int)(wc & 7) - 2);
+  const int ny = pty - ((int)(wc >> 3) - 2);
+  ix = -nx;
+  iy = -ny;
+  return nx >= -1 && nx <= 1 && ny >= -1 && ny <= 1;
+}
+
+// Exact kNN of one query by the whole wavefront, starting from radius R.
+// Writes the k entries of `row` (BITWISE: sorted descending by (d2, ID,
+// image), the reference's queue order) and returns tau and the density
+// kernel sum (BITWISE: summed farthest first, sim/sph.go:313-320).
+template <bool BITWISE, bool FP32C>
+__device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double yq, double R,
+                                 const double2* __restrict__ XY,
+                                 const uint8_t* __restrict__ WR,
+                                 const uint32_t* __restrict__ cstart,
+                                 const uint32_t* __restrict__ ID, uint32_t* __restrict__ row,
+                                 SearchShared& sh, unsigned long long& nretry,
+                                 unsigned long long& nties, unsigned long long& nfail) {
+  const int lane = lane_id();
+  const int k = S->k;
+  const double lx = S->lx, ly = S->ly;
+  const double rcap = 1.5 * (lx > ly ? lx : ly);
+  if (R > rcap) R = rcap;
+  double r_lo = 0.0, r_hi = 0.0;
+  int cnt = 0;
+  double R2 = 0;
+  int attempt = 0;
+  for (;; ++attempt) {
+    R2 = R * R;
+    const double Rm = R * (1 + 1e-12) + 1e-15 * (lx + ly);
+    const int c = level_for(S, 2 * Rm, 2 * Rm, 8.0);
+    const int64_t nc = 1LL << c;
+    const double icx = (double)nc / lx, icy = (double)nc / ly;
+    const double scx = lx / (double)nc, scy = ly / (double)nc;
+    int64_t X0 = (int64_t)floor((xq - S->x0 - Rm) * icx), X1 = (int64_t)floor((xq - S->x0 + Rm) * icx);
+    int64_t Y0 = (int64_t)floor((yq - S->y0 - Rm) * icy), Y1 = (int64_t)floor((yq - S->y0 + Rm) * icy);
+    if (X0 < -2 * nc) X0 = -2 * nc;
+    if (Y0 < -2 * nc) Y0 = -2 * nc;
+    if (X1 > 3 * nc - 1) X1 = 3 * nc - 1;
+    if (Y1 > 3 * nc - 1) Y1 = 3 * nc - 1;
+    const int nX = (int)(X1 - X0 + 1), nY = (int)(Y1 - Y0 + 1);
+    const int ncell = nX * nY;
+    int npieces = 0, ncand = 0;
+    for (int cb = 0; cb < ncell; cb += 64) {
+      const int ci = cb + lane;
+      bool have = false;
+      uint32_t a = 0, b = 0;
+      int tx = 0, ty = 0;
+      if (ci < ncell) {
+        const int64_t Xc = X0 + ci % nX, Yc = Y0 + ci / nX;
+        const double bx0 = S->x0 + (double)Xc * scx, by0 = S->y0 + (double)Yc * scy;
+        const double dx = xq < bx0 ? bx0 - xq : (xq > bx0 + scx ? xq - (bx0 + scx) : 0.0);
+        const double dy = yq < by0 ? by0 - yq : (yq > by0 + scy ? yq - (by0 + scy) : 0.0);
+        if (dx * dx + dy * dy <= Rm * Rm) {
+          tx = (int)floordiv(Xc, nc);
+          ty = (int)floordiv(Yc, nc);
+          cell_range(S, cstart, c, (uint32_t)(Xc - tx * nc), (uint32_t)(Yc - ty * nc), a, b);
+          have = b > a;
+        }
+      }
+      const unsigned long long m = ballot(have);
+      const int len = have ? (int)(b - a) : 0;
+      const int incl = wave_incl_scan(len);
+      if (have) {
+        const int pos = npieces + lane_prefix(m);
+        sh.pst[pos] = a;
+        sh.ppref[pos] = ncand + incl - len;
+        sh.ptx[pos] = (int8_t)tx;
+        sh.pty[pos] = (int8_t)ty;
+      }
+      npieces += __popcll(m);
+      ncand += __shfl(incl, 63, 64);
+    }
+    if (lane == 0) sh.ppref[npieces] = ncand;
+    __builtin_amdgcn_wave_barrier();
+    int nsteps = 0;
+    while ((1 << nsteps) < npieces) ++nsteps;
+    cnt = 0;
+    for (int base = 0; base < ncand; base += 64) {
+      const int u = base + lane;
+      bool acc = false;
+      double d2 = 0;
+      uint32_t pk = 0;
+      if (u < ncand) {
+        int lo = 0, hi = npieces - 1;
+        for (int s = 0; s < nsteps; ++s) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sh.ppref[mid] <= u) lo = mid; else hi = mid - 1;
+        }
+        const uint32_t j = sh.pst[lo] + (uint32_t)(u - sh.ppref[lo]);
+        int ix, iy;
+        if ((int64_t)j != t && cand_image(sh.ptx[lo], sh.pty[lo], WR[j], ix, iy)) {
+          const double ox = (double)ix * lx, oy = (double)iy * ly;
+          const double2 pj = XY[j];
+          const double ex = (xq + ox) - pj.x;
+          const double ey = (yq + oy) - pj.y;
+          d2 = ex * ex + ey * ey;
+          acc = d2 <= R2;
+          pk = (j << 4) | img_code(ix, iy);
+        }
+      }
+      const unsigned long long m = ballot(acc);
+      if (acc) {
+        const int pos = cnt + lane_prefix(m);
+        if (pos < kCandCap) {
+          sh.d2[pos] = d2;
+          sh.pk[pos] = pk;
+        }
+      }
+      cnt += __popcll(m);
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (cnt >= k && cnt <= kCandCap) break;
+    if (cnt < k) {
+      r_lo = R;
+      double f = cnt > 0 ? sqrt((1.5 * k) / cnt) : 2.0;
+      if (f < 1.25) f = 1.25;
+      if (f > 4.0) f = 4.0;
+      R = R * f;
+      if (r_hi > 0 && R >= r_hi) R = sqrt(r_lo * r_hi);
+      if (R > rcap) R = rcap;
+    } else {
+      r_hi = R;
+      double f = sqrt((0.5 * (k + kCandCap)) / cnt);
+      if (f > 0.9) f = 0.9;
+      R = R * f;
+      if (R <= r_lo) R = sqrt(r_lo * r_hi);
+    }
+    if (attempt > 60 || (cnt < k && r_lo >= rcap)) {
+      if (lane == 0) ++nfail;
+      break;
+    }
+  }
+  if (attempt > 0 && lane == 0) ++nretry;
+  SearchOut out;
+  out.tau = 0;
+  out.facc = 0;
+  if (cnt < k || cnt > kCandCap) return out;
+  // k-th smallest squared distance: ballot quickselect
+  constexpr int NS = kCandCap / 64;
+  double v[NS];
+  uint32_t pkr[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int idx = s * 64 + lane;
+    v[s] = idx < cnt ? sh.d2[idx] : INFINITY;
+    pkr[s] = idx < cnt ? sh.pk[idx] : 0u;
+  }
+  double lo = -1.0, hi = INFINITY, tau = 0;
+  int c_lt = 0, c_le = 0;
+  for (;;) {
+    int slot = -1, src = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const unsigned long long b = ballot(v[s] > lo && v[s] < hi);
+      if (slot < 0 && b) {
+        slot = s;
+        src = __builtin_ctzll(b);
+      }
+    }
+    double pv = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s)
+      if (s == slot) pv = readlane_d(v[s], src);
+    c_lt = 0;
+    c_le = 0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      c_lt += __popcll(ballot(v[s] < pv));
+      c_le += __popcll(ballot(v[s] <= pv));
+    }
+    if (c_lt < k && k <= c_le) {
+      tau = pv;
+      break;
+    }
+    if (k <= c_lt) hi = pv; else lo = pv;
+  }
+  // ties at tau keep the smallest (ID, image)
+  bool tsel[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) tsel[s] = false;
+  if (c_le > k) {
+    if (lane == 0) ++nties;
+    uint64_t key2[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int idx = s * 64 + lane;
+      key2[s] = ~0ull;
+      if (idx < cnt && v[s] == tau)
+        key2[s] = ((uint64_t)ID[pkr[s] >> 4] << 4) | img_order(pkr[s] & 15u);
+    }
+    for (int r = 0; r < k - c_lt; ++r) {
+      uint64_t mn = ~0ull;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) mn = key2[s] < mn ? key2[s] : mn;
+      for (int o = 32; o > 0; o >>= 1) {
+        const uint64_t other = __shfl_xor(mn, o, 64);
+        mn = other < mn ? other : mn;
+      }
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+        if (key2[s] == mn) {
+          tsel[s] = true;
+          key2[s] = ~0ull;
+        }
+    }
+  }
+  const double h = sqrt(tau);
+  const int kern = S->kernel;
+  double facc = 0.0;
+  float fs = 0.f, fc = 0.f;
+  const float hinv_f = 1.0f / (float)h;
+  int emitted = 0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    const int idx = s * 64 + lane;
+    const bool sel = idx < cnt && (v[s] < tau || (v[s] == tau && (c_le == k || tsel[s])));
+    const unsigned long long m = ballot(sel);
+    if (sel) {
+      const int pos = emitted + lane_prefix(m);
+      if constexpr (!BITWISE) {
+        row[pos] = pkr[s];
+        if constexpr (FP32C) {
+          const float q = sqrtf((float)v[s]) * hinv_f;
+          const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
+          const float y = f - fc;
+          const float tt = fs + y;
+          fc = (tt - fs) - y;
+          fs = tt;
+        } else {
+          facc += kern_F(kern, sqrt(v[s]) / h);
+        }
+      } else {
+        sh.sd2[pos] = v[s];
+        sh.spk[pos] = pkr[s];
+      }
+    }
+    emitted += __popcll(m);
+  }
+  if constexpr (!BITWISE) {
+    if constexpr (FP32C) {
+      // compensated tree sum: each level carries its rounding error
+      float s_ = fs, c_ = -fc;
+      for (int o = 32; o > 0; o >>= 1) {
+        const float so = __shfl_xor(s_, o, 64);
+        const float co = __shfl_xor(c_, o, 64);
+        const float sum = s_ + so;
+        const float bp = sum - s_;
+        const float err = (s_ - (sum - bp)) + (so - bp);
+        s_ = sum;
+        c_ = c_ + co + err;
+      }
+      out.facc = (double)s_ + (double)c_;
+    } else {
+      out.facc = group_sum(facc, 64);
+    }
+  } else {
+    __builtin_amdgcn_wave_barrier();
+    double dv = lane < k ? sh.sd2[lane] : -1.0;
+    uint32_t pv = lane < k ? sh.spk[lane] : 0u;
+    uint32_t iv = lane < k ? ID[pv >> 4] : 0u;
+    // bitonic sort, descending by (d2, ID, image)
+    for (int size = 2; size <= 64; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        const double od = __shfl_xor(dv, stride, 64);
+        const uint32_t op = __shfl_xor(pv, stride, 64);
+        const uint32_t oi = __shfl_xor(iv, stride, 64);
+        const bool other_greater =
+            od > dv || (od == dv && (oi > iv || (oi == iv && img_order(op & 15u) > img_order(pv & 15u))));
+        const bool lower = (lane & stride) == 0;
+        const bool desc = (lane & size) == 0;
+        const bool take = (lower == desc) ? other_greater : !other_greater;
+        if (take && (od != dv || oi != iv || op != pv)) {
+          dv = od;
+          pv = op;
+          iv = oi;
+        }
+      }
+    }
+    if (lane < k) {
+      row[lane] = pv;
+      sh.fterm[lane] = kern_F(kern, sqrt(dv) / h);
+    }
+    __builtin_amdgcn_wave_barrier();
+    double acc = 0.0;
+    for (int n = 0; n < k; ++n) acc += sh.fterm[n];
+    out.facc = acc;
+  }
+  out.tau = tau;
+  __builtin_amdgcn_wave_barrier();
+  return out;
+}
+
+// Density, sound speed, h and the force record of query t.
+template <bool FP32C>
+__device__ inline void write_result(const SimDev* S, int64_t t, double xq, double yq, double tau,
+                                    double facc, const double* __restrict__ EP,
+                                    const double* __restrict__ VPX,
+                                    const double* __restrict__ VPY, double* __restrict__ H,
+                                    double* __restrict__ RHO, double* __restrict__ C,
+                                    void* __restrict__ recp) {
+  const double h = sqrt(tau);
+  const double rho = S->fpref * S->mass * facc / (h * h);
+  const double c = sqrt(S->cfac * EP[t]);
+  H[t] = h;
+  RHO[t] = rho;
+  C[t] = c;
+  const double B = c * c / (S->gamma * rho);
+  if constexpr (FP32C) {
+    RecF r;
+    r.x = xq;
+    r.y = yq;
+    r.vpx = (float)VPX[t];
+    r.vpy = (float)VPY[t];
+    r.rho = (float)rho;
+    r.c = (float)c;
+    r.h = (float)h;
+    r.B = (float)B;
+    r.pad0 = 0.f;
+    r.pad1 = 0.f;
+    ((RecF*)recp)[t] = r;
+  } else {
+    RecD r;
+    r.x = xq;
+    r.y = yq;
+    r.vpx = VPX[t];
+    r.vpy = VPY[t];
+    r.rho = rho;
+    r.c = c;
+    r.h = h;
+    r.B = B;
+    ((RecD*)recp)[t] = r;
+  }
+}
+
+// First-pass search radius: local density from the occupancy of the
+// query's cell at a level holding a few particles.
+__device__ inline double local_radius(const SimDev* S, const uint32_t* __restrict__ cstart,
+                                      double xq, double yq) {
+  const CellIdx ci = cell_of(S, xq, yq);
+  const int64_t fx = ci.ux - floordiv(ci.ux, S->gx) * S->gx;
+  const int64_t fy = ci.uy - floordiv(ci.uy, S->gy) * S->gy;
+  int c = S->lg - 1 > 0 ? S->lg - 1 : 0;
+  for (;; --c) {
+    const int sh = S->lg - c;
+    uint32_t a, b;
+    cell_range(S, cstart, c, (uint32_t)(fx >> sh), (uint32_t)(fy >> sh), a, b);
+    const double area = (S->lx / (double)(1LL << c)) * (S->ly / (double)(1LL << c));
+    if (b - a >= 8 || c == 0) {
+      const double dens = (double)(b - a > 0 ? b - a : 1) / area;
+      const double r = sqrt(1.3 * S->k / (M_PI * dens));
+      return r < S->r_init * 4 ? r : S->r_init * 4;
+    }
+  }
+}
+
+// kNN + density + sound speed + force record.  A wave takes 64 consecutive
+// (Morton-ordered) queries.  Fast path (lane = query): the wave stages the
+// candidates of the tile's bounding box (grown by each query's bracket
+// radius) through LDS 64 at a time and every lane tests all of them against
+// its bracket [h_prev (1 - delta), h_prev (1 + delta)]: nearer candidates
+// are certain neighbours and go straight to the list; the annulus keeps at
+// most kAnn entries from which the remaining ones are selected exactly.
+// Queries without a previous h, or whose bracket misses, fall back to the
+// exact wavefront search.  BITWISE uses the wavefront search for all.
 template <bool BITWISE, bool FP32C>
 __global__ __launch_bounds__(256) void k_knn(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
-    const double* __restrict__ X, const double* __restrict__ Y,
+    const double2* __restrict__ XY,
     const uint8_t* __restrict__ WR, const uint32_t* __restrict__ cstart,
-    double* __restrict__ H, const double* __restrict__ EP, const uint32_t* __restrict__ ID,
+    double* __restrict__ H, const double* __restrict__ EP, const double* __restrict__ VPX,
+    const double* __restrict__ VPY, const uint32_t* __restrict__ ID,
     uint32_t* __restrict__ nbr, int kstride, double* __restrict__ RHO,
-    double* __restrict__ C, KnnStats* __restrict__ stats) {
-  __shared__ KnnShared sh_all[4];
-  const Item it = items[xcd_item(nitems)];
+    double* __restrict__ C, void* __restrict__ recp, KnnStats* __restrict__ stats,
+    uint2* __restrict__ retry, uint32_t* __restrict__ retry_count) {
+  __shared__ TileShared wsh[4];
+  const int item_idx = xcd_item(nitems);
+  const Item it = items[item_idx];
   const SimDev* S = sims + it.sim;
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  KnnShared& sh = sh_all[wv];
+  TileShared& ts = wsh[wv];
   const int k = S->k;
-  const int32_t gx = S->gx, gy = S->gy;
   const double lx = S->lx, ly = S->ly;
-  unsigned long long nretry = 0, nties = 0, nfail = 0;
+  unsigned long long nretry = 0, nties = 0, nfail = 0, nfall = 0;
+  unsigned long long nmiss_lo = 0, nmiss_hi = 0, nmiss_ann = 0, nsplitfb = 0;
 
-  for (int qq = 0; qq < 64; ++qq) {
-    const int qi = wv * 64 + qq;
-    if (qi >= it.count) break;
-    const int64_t t = it.first + qi;
-    const double xq = X[t], yq = Y[t];
-    const double hp = H[t];
-    double R = hp > 0 ? hp * 1.1 : S->r_init;
-    const double rcap = 1.5 * (lx > ly ? lx : ly);
-    if (R > rcap) R = rcap;
-    double r_lo = 0.0, r_hi = 0.0;  // bracket: count < k at r_lo, overflow at r_hi
-    int cnt = 0;
-    double R2 = 0;
-    int attempt = 0;
-    for (;; ++attempt) {
-      R2 = R * R;
-      // culling radius: a hair larger than R so that binning rounding can
-      // never hide a particle with d2 <= R^2
-      const double Rm = R * (1 + 1e-12) + 1e-15 * (lx + ly);
-      const double Rm2 = Rm * Rm;
-      int64_t ry0 = (int64_t)floor((yq - S->y0 - Rm) * S->ivy);
-      int64_t ry1 = (int64_t)floor((yq - S->y0 + Rm) * S->ivy);
-      if (ry0 < -2LL * gy) ry0 = -2LL * gy;
-      if (ry1 > 3LL * gy - 1) ry1 = 3LL * gy - 1;
-      cnt = 0;
-      for (int64_t rb = ry0; rb <= ry1; rb += 64) {
-        // ---- pieces of this lane's row ----
-        const int64_t ry = rb + lane;
-        uint32_t pst[5];
-        int plen[5];
-        int ptx[5];
-        int np = 0;
-        int pty = 0;
-        if (ry <= ry1) {
-          const double by0 = S->y0 + (double)ry * S->sy;
-          const double by1 = by0 + S->sy;
-          const double dy = yq < by0 ? by0 - yq : (yq > by1 ? yq - by1 : 0.0);
-          const double w2 = Rm2 - dy * dy;
-          if (w2 >= 0) {
-            const double w = sqrt(w2);
-            int64_t cx0 = (int64_t)floor((xq - S->x0 - w) * S->ivx);
-            int64_t cx1 = (int64_t)floor((xq - S->x0 + w) * S->ivx);
-            if (cx0 < -2LL * gx) cx0 = -2LL * gx;
-            if (cx1 > 3LL * gx - 1) cx1 = 3LL * gx - 1;
-            const int64_t ty = floordiv(ry, gy);
-            pty = (int)ty;
-            const int64_t rowbase = S->coff + (ry - ty * gy) * gx;
-            int64_t c = cx0;
-#pragma unroll
-            for (int p = 0; p < 5; ++p) {
-              if (c <= cx1) {
-                const int64_t tx = floordiv(c, gx);
-                int64_t cend = (tx + 1) * gx - 1;
-                if (cend > cx1) cend = cx1;
-                const uint32_t a = cstart[rowbase + (c - tx * gx)];
-                const uint32_t b = cstart[rowbase + (cend - tx * gx) + 1];
-                if (b > a) {
-                  pst[np] = a;
-                  plen[np] = (int)(b - a);
-                  ptx[np] = (int)tx;
-                  ++np;
-                }
-                c = cend + 1;
-              }
-            }
-          }
+  const int qi = wv * 64 + lane;
+  const bool valid = qi < it.count;
+  const int64_t t = it.first + (valid ? qi : 0);
+  const double2 pq = XY[t];
+  const double xq = pq.x, yq = pq.y;
+  const double hp = H[t];
+  uint32_t* row = nbr + t * kstride;
+  bool fast = !BITWISE && valid && hp > 0;
+  bool resolved = false;
+  double tau = 0.0;
+  if constexpr (!BITWISE) {
+    const double Rlo = hp * (1 - kDelta), Rhi = hp * (1 + kDelta);
+    const double Rlo2 = Rlo * Rlo, Rhi2 = Rhi * Rhi;
+    const double Rm = Rhi * (1 + 1e-12) + 1e-15 * (lx + ly);
+    int c_lo = 0, c_an = 0;
+    const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
+    const uint32_t tu = (uint32_t)t;
+    // Lane groups: aligned blocks of consecutive queries.  A block whose
+    // box holds more than kSplit candidates per query is halved (Morton
+    // order can put two distant regions in one wave); blocks below 8 lanes
+    // go to the wavefront search instead.
+    unsigned long long pending = ballot(fast);
+    int gsz = 64;
+    while (pending) {
+      const int l0 = __builtin_ctzll(pending);
+      const int ga = l0 & ~(gsz - 1);
+      const bool grp = fast && ((pending >> lane) & 1ull) && lane >= ga && lane < ga + gsz;
+      const unsigned long long gm = ballot(grp);
+      double bx0 = grp ? xq - Rm : INFINITY, bx1 = grp ? xq + Rm : -INFINITY;
+      double by0 = grp ? yq - Rm : INFINITY, by1 = grp ? yq + Rm : -INFINITY;
+      for (int o = 32; o > 0; o >>= 1) {
+        bx0 = fmin(bx0, __shfl_xor(bx0, o, 64));
+        by0 = fmin(by0, __shfl_xor(by0, o, 64));
+        bx1 = fmax(bx1, __shfl_xor(bx1, o, 64));
+        by1 = fmax(by1, __shfl_xor(by1, o, 64));
+      }
+      const int c = level_for(S, bx1 - bx0, by1 - by0, 14.0);
+      const int64_t nc = 1LL << c;
+      const double icx = (double)nc / lx, icy = (double)nc / ly;
+      int64_t X0 = (int64_t)floor((bx0 - S->x0) * icx), X1 = (int64_t)floor((bx1 - S->x0) * icx);
+      int64_t Y0 = (int64_t)floor((by0 - S->y0) * icy), Y1 = (int64_t)floor((by1 - S->y0) * icy);
+      if (X0 < -2 * nc) X0 = -2 * nc;
+      if (Y0 < -2 * nc) Y0 = -2 * nc;
+      if (X1 > 3 * nc - 1) X1 = 3 * nc - 1;
+      if (Y1 > 3 * nc - 1) Y1 = 3 * nc - 1;
+      const int nX = (int)(X1 - X0 + 1), nY = (int)(Y1 - Y0 + 1);
+      const int ncell = nX * nY;
+      int npieces = 0, ncand = 0;
+      for (int cb = 0; cb < ncell; cb += 64) {
+        const int ci = cb + lane;
+        bool have = false;
+        uint32_t pa = 0, pb = 0;
+        int tx = 0, ty = 0;
+        if (ci < ncell) {
+          const int64_t Xc = X0 + ci % nX, Yc = Y0 + ci / nX;
+          tx = (int)floordiv(Xc, nc);
+          ty = (int)floordiv(Yc, nc);
+          cell_range(S, cstart, c, (uint32_t)(Xc - tx * nc), (uint32_t)(Yc - ty * nc), pa, pb);
+          have = pb > pa;
         }
-        int tot = 0;
-#pragma unroll
-        for (int p = 0; p < 5; ++p) tot += p < np ? plen[p] : 0;
-        const int pbase_incl = wave_incl_scan(np);
-        const int cbase_incl = wave_incl_scan(tot);
-        const int npieces = __shfl(pbase_incl, 63, 64);
-        const int ncand = __shfl(cbase_incl, 63, 64);
-        {
-          int pi = pbase_incl - np;
-          int ci = cbase_incl - tot;
-#pragma unroll
-          for (int p = 0; p < 5; ++p) {
-            if (p < np) {
-              sh.pst[pi] = pst[p];
-              sh.ppref[pi] = ci;
-              sh.ptx[pi] = (int8_t)ptx[p];
-              sh.pty[pi] = (int8_t)pty;
-              ++pi;
-              ci += plen[p];
-            }
+        const unsigned long long m = ballot(have);
+        const int len = have ? (int)(pb - pa) : 0;
+        const int incl = wave_incl_scan(len);
+        if (have) {
+          const int pos = npieces + lane_prefix(m);
+          ts.pst[pos] = pa;
+          ts.ppref[pos] = ncand + incl - len;
+          ts.ptx[pos] = (int8_t)tx;
+          ts.pty[pos] = (int8_t)ty;
+        }
+        npieces += __popcll(m);
+        ncand += __shfl(incl, 63, 64);
+      }
+      npieces = __builtin_amdgcn_readfirstlane(npieces);
+      ncand = __builtin_amdgcn_readfirstlane(ncand);
+      const int gs = __popcll(gm);
+      if (ncand > kSplit * gs) {
+        if (gsz > 8) {
+          gsz >>= 1;
+        } else {
+          if (grp) {
+            fast = false;
+            ++nsplitfb;
           }
-          if (lane == 63) sh.ppref[npieces] = ncand;
+          pending &= ~gm;
+          gsz = 64;
         }
         __builtin_amdgcn_wave_barrier();
-        int nsteps = 0;
-        while ((1 << nsteps) < npieces) ++nsteps;
-        // ---- walk the candidates ----
-        for (int base = 0; base < ncand; base += 64) {
-          const int u = base + lane;
-          bool acc = false;
-          double d2 = 0;
-          uint32_t pk = 0;
-          if (u < ncand) {
-            int lo = 0, hi = npieces - 1;
-            for (int s = 0; s < nsteps; ++s) {
-              const int mid = (lo + hi + 1) >> 1;
-              if (sh.ppref[mid] <= u) lo = mid; else hi = mid - 1;
-            }
-            const uint32_t j = sh.pst[lo] + (uint32_t)(u - sh.ppref[lo]);
-            const uint8_t wc = WR[j];
-            const int nx = (int)sh.ptx[lo] - ((int)(wc & 7) - 2);
-            const int ny = (int)sh.pty[lo] - ((int)(wc >> 3) - 2);
-            if ((int64_t)j != t && nx >= -1 && nx <= 1 && ny >= -1 && ny <= 1) {
-              // query offset o = -n * L, the reference's float64(i)*deltaX
-              const double ox = (double)(-nx) * lx, oy = (double)(-ny) * ly;
-              const double ex = (xq + ox) - X[j];
-              const double ey = (yq + oy) - Y[j];
-              d2 = ex * ex + ey * ey;
-              acc = d2 <= R2;
-              pk = (j << 4) | (uint32_t)((-nx + 1) * 3 + (-ny + 1));
-            }
+        continue;
+      }
+      if (lane == 0) ts.ppref[npieces] = ncand;
+      __builtin_amdgcn_wave_barrier();
+      int nsteps = 0;
+      while ((1 << nsteps) < npieces) ++nsteps;
+      for (int base = 0; base < ncand; base += 64) {
+        // stage 64 candidates
+        const int u = base + lane;
+        double cxv = NAN, cyv = 0.0;
+        uint32_t cpk = 0;
+        bool interior = true;
+        if (u < ncand) {
+          int lo = 0, hi = npieces - 1;
+          for (int s = 0; s < nsteps; ++s) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ts.ppref[mid] <= u) lo = mid; else hi = mid - 1;
           }
-          const unsigned long long m = ballot(acc);
-          if (acc) {
-            const int pos = cnt + lane_prefix(m);
-            if (pos < kCandCap) {
-              sh.d2[pos] = d2;
-              sh.pk[pos] = pk;
-            }
+          const uint32_t j = ts.pst[lo] + (uint32_t)(u - ts.ppref[lo]);
+          int ix, iy;
+          if (cand_image(ts.ptx[lo], ts.pty[lo], WR[j], ix, iy)) {
+            const double2 pj = XY[j];
+            cxv = pj.x;
+            cyv = pj.y;
+            interior = (ix == 0) && (iy == 0);
           }
-          cnt += __popcll(m);
+          cpk = (j << 4) | img_code(ix, iy);
         }
+        ts.sxy[lane] = make_double2(cxv, cyv);
+        ts.spk[lane] = cpk;
+        const bool all_interior = ballot(interior) == ~0ull;
         __builtin_amdgcn_wave_barrier();
-      }
-      if (cnt >= k && cnt <= kCandCap) break;
-      // new radius from the area law, kept inside the bracket
-      if (cnt < k) {
-        r_lo = R;
-        double f = cnt > 0 ? sqrt((1.5 * k) / cnt) : 2.0;
-        if (f < 1.25) f = 1.25;
-        if (f > 4.0) f = 4.0;
-        R = R * f;
-        if (r_hi > 0 && R >= r_hi) R = sqrt(r_lo * r_hi);
-        if (R > rcap) R = rcap;
-      } else {
-        r_hi = R;
-        double f = sqrt((0.5 * (k + kCandCap)) / cnt);
-        if (f > 0.9) f = 0.9;
-        R = R * f;
-        if (R <= r_lo) R = sqrt(r_lo * r_hi);
-      }
-      if (attempt > 60 || (cnt < k && r_lo >= rcap)) {
-        ++nfail;
-        break;
-      }
-    }
-    if (attempt > 0) ++nretry;
-    if (cnt < k || cnt > kCandCap) {
-      // unreachable for valid input: record, emit an empty pass result
-      if (lane == 0) {
-        H[t] = 0.0;
-        RHO[t] = 0.0;
-        C[t] = 0.0;
-      }
-      continue;
-    }
-    // ---- k-th smallest squared distance: ballot quickselect ----
-    double v[kCandCap / 64];
-    uint32_t pkr[kCandCap / 64];
-#pragma unroll
-    for (int s = 0; s < kCandCap / 64; ++s) {
-      const int idx = s * 64 + lane;
-      v[s] = idx < cnt ? sh.d2[idx] : INFINITY;
-      pkr[s] = idx < cnt ? sh.pk[idx] : 0u;
-    }
-    double lo = -1.0, hi = INFINITY, tau = 0;
-    int c_lt = 0, c_le = 0;
-    for (;;) {
-      int slot = -1, src = 0;
-#pragma unroll
-      for (int s = 0; s < kCandCap / 64; ++s) {
-        const unsigned long long b = ballot(v[s] > lo && v[s] < hi);
-        if (slot < 0 && b) {
-          slot = s;
-          src = __builtin_ctzll(b);
-        }
-      }
-      double pv = 0;
-#pragma unroll
-      for (int s = 0; s < kCandCap / 64; ++s)
-        if (s == slot) pv = readlane_d(v[s], src);
-      c_lt = 0;
-      c_le = 0;
-#pragma unroll
-      for (int s = 0; s < kCandCap / 64; ++s) {
-        c_lt += __popcll(ballot(v[s] < pv));
-        c_le += __popcll(ballot(v[s] <= pv));
-      }
-      if (c_lt < k && k <= c_le) {
-        tau = pv;
-        break;
-      }
-      if (k <= c_lt) hi = pv; else lo = pv;
-    }
-    // ---- ties at tau: keep the smallest (ID, image) ----
-    bool tsel[kCandCap / 64];
-#pragma unroll
-    for (int s = 0; s < kCandCap / 64; ++s) tsel[s] = false;
-    if (c_le > k) {
-      ++nties;
-      uint64_t key2[kCandCap / 64];
-#pragma unroll
-      for (int s = 0; s < kCandCap / 64; ++s) {
-        const int idx = s * 64 + lane;
-        key2[s] = ~0ull;
-        if (idx < cnt && v[s] == tau)
-          key2[s] = ((uint64_t)ID[pkr[s] >> 4] << 4) | (pkr[s] & 15u);
-      }
-      for (int r = 0; r < k - c_lt; ++r) {
-        uint64_t mn = ~0ull;
-#pragma unroll
-        for (int s = 0; s < kCandCap / 64; ++s) mn = key2[s] < mn ? key2[s] : mn;
-        for (int o = 32; o > 0; o >>= 1) {
-          const uint64_t other = __shfl_xor(mn, o, 64);
-          mn = other < mn ? other : mn;
-        }
-#pragma unroll
-        for (int s = 0; s < kCandCap / 64; ++s)
-          if (key2[s] == mn) {
-            tsel[s] = true;
-            key2[s] = ~0ull;
-          }
-      }
-    }
-    // ---- emit the neighbour list, density, sound speed ----
-    const double h = sqrt(tau);
-    const int kern = S->kernel;
-    double facc = 0.0;
-    float fs = 0.f, fc = 0.f;
-    const float hinv_f = 1.0f / (float)h;
-    int emitted = 0;
-#pragma unroll
-    for (int s = 0; s < kCandCap / 64; ++s) {
-      const int idx = s * 64 + lane;
-      const bool sel = idx < cnt && (v[s] < tau || (v[s] == tau && (c_le == k || tsel[s])));
-      const unsigned long long m = ballot(sel);
-      if (sel) {
-        const int pos = emitted + lane_prefix(m);
-        const uint32_t pkv = pkr[s];
-        if constexpr (!BITWISE) {
-          nbr[t * kstride + pos] = pkv;
-          if constexpr (FP32C) {
-            const float q = sqrtf((float)v[s]) * hinv_f;
-            const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
-            // Kahan-compensated accumulation of this lane's terms
-            const float y = f - fc;
-            const float tt = fs + y;
-            fc = (tt - fs) - y;
-            fs = tt;
-          } else {
-            facc += kern_F(kern, sqrt(v[s]) / h);
+        // entries past the last candidate hold NaN and are rejected
+        const int nm = (__builtin_amdgcn_readfirstlane(ncand - base < 64 ? ncand - base : 64) + 3) & ~3;
+#define SPH_ACCEPT(D2, PK)                                                              \
+  {                                                                                     \
+    const double d2v_ = (D2);                                                           \
+    const uint32_t pkv_ = (PK);                                                         \
+    const bool ok_ = grp && d2v_ <= Rhi2 && (pkv_ >> 4) != tu;                          \
+    const bool isc_ = ok_ && d2v_ < Rlo2;                                               \
+    const bool isa_ = ok_ && !(d2v_ < Rlo2);                                            \
+    if (isc_ && c_lo < k) row[c_lo] = pkv_;                                             \
+    if (isa_ && c_an < kAnn) ts.ann_pk[c_an][lane] = pkv_;                              \
+    c_lo += isc_ ? 1 : 0;                                                               \
+    c_an += isa_ ? 1 : 0;                                                               \
+  }
+        if (all_interior) {
+          for (int m = 0; m < nm; m += 4) {
+            const double2 e0 = ts.sxy[m], e1 = ts.sxy[m + 1], e2 = ts.sxy[m + 2], e3 = ts.sxy[m + 3];
+            const u32x4 p4 = *(const u32x4*)&ts.spk[m];
+            double a0 = xq - e0.x, b0 = yq - e0.y;
+            double a1 = xq - e1.x, b1 = yq - e1.y;
+            double a2 = xq - e2.x, b2 = yq - e2.y;
+            double a3 = xq - e3.x, b3 = yq - e3.y;
+            const double d0 = a0 * a0 + b0 * b0, d1 = a1 * a1 + b1 * b1;
+            const double d2_ = a2 * a2 + b2 * b2, d3 = a3 * a3 + b3 * b3;
+            SPH_ACCEPT(d0, p4.x);
+            SPH_ACCEPT(d1, p4.y);
+            SPH_ACCEPT(d2_, p4.z);
+            SPH_ACCEPT(d3, p4.w);
           }
         } else {
-          sh.sd2[pos] = v[s];  // staged for the ordered sum
-          sh.spk[pos] = pkv;
+          for (int m = 0; m < nm; ++m) {
+            const uint32_t pk = ts.spk[m];
+            const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
+            const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
+            const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
+            const double2 e = ts.sxy[m];
+            const double ex = ax - e.x;
+            const double ey = ay - e.y;
+            SPH_ACCEPT(ex * ex + ey * ey, pk);
+          }
         }
+#undef SPH_ACCEPT
+        __builtin_amdgcn_wave_barrier();
       }
-      emitted += __popcll(m);
+      pending &= ~gm;
+      gsz = 64;
     }
-    double rho;
-    if constexpr (!BITWISE) {
-      double tot;
-      if constexpr (FP32C) {
-        // compensated tree sum: carry each level's rounding error
-        float s_ = fs, c_ = -fc;
-        for (int o = 32; o > 0; o >>= 1) {
-          const float so = __shfl_xor(s_, o, 64);
-          const float co = __shfl_xor(c_, o, 64);
-          const float sum = s_ + so;
-          const float bp = sum - s_;
-          const float err = (s_ - (sum - bp)) + (so - bp);
-          s_ = sum;
-          c_ = c_ + co + err;
-        }
-        tot = (double)s_ + (double)c_;
-      } else {
-        tot = group_sum(facc, 64);
+    if (ballot(fast)) {
+      // resolve: the r = k - c_lo smallest annulus entries by (d2, ID, image)
+      if (fast) {
+        if (c_lo >= k) ++nmiss_lo;
+        else if (c_an > kAnn) ++nmiss_ann;
+        else if (c_lo + c_an < k) ++nmiss_hi;
       }
-      rho = S->fpref * S->mass * tot / (h * h);
-    } else {
-      // reference order: queue index 0 = k-th (farthest) first,
-      // sim/sph.go:313-320; sort the k entries descending by (d2, ID)
+      if (fast && c_lo < k && c_an <= kAnn && c_lo + c_an >= k) {
+        const int r = k - c_lo;
+        auto keyof = [&](uint32_t pk) {
+          return ((uint64_t)ID[pk >> 4] << 4) | img_order(pk & 15u);
+        };
+        // annulus squared distances, recomputed exactly as in the test
+        double ad[kAnn];
+        uint32_t ap[kAnn];
+#pragma unroll
+        for (int e = 0; e < kAnn; ++e) {
+          ad[e] = INFINITY;
+          ap[e] = 0;
+          if (e < c_an) {
+            const uint32_t pk = ts.ann_pk[e][lane];
+            const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
+            const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
+            const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
+            const double2 pj = XY[pk >> 4];
+            const double ex = ax - pj.x, ey = ay - pj.y;
+            ad[e] = ex * ex + ey * ey;
+            ap[e] = pk;
+          }
+        }
+        double pd = -1.0;
+        uint32_t ppk = 0;
+        for (int s = 0; s < r; ++s) {
+          double bd = INFINITY;
+          uint32_t bpk = 0;
+#pragma unroll
+          for (int e = 0; e < kAnn; ++e) {
+            const double d = ad[e];
+            const uint32_t pk = ap[e];
+            bool take = e < c_an && d >= pd && d <= bd;
+            if (take && d == pd) take = keyof(pk) > keyof(ppk);
+            if (take && d == bd) take = keyof(pk) < keyof(bpk);
+            if (take) {
+              bd = d;
+              bpk = pk;
+            }
+          }
+          row[c_lo + s] = bpk;
+          pd = bd;
+          ppk = bpk;
+        }
+        tau = pd;
+        // a tie at the boundary if an unpicked annulus entry equals tau
+#pragma unroll
+        for (int e = 0; e < kAnn; ++e)
+          if (e < c_an && ad[e] == tau && keyof(ap[e]) > keyof(ppk)) ++nties;
+        resolved = true;
+      }
       __builtin_amdgcn_wave_barrier();
-      double dv = lane < k ? sh.sd2[lane] : -1.0;
-      uint32_t pv = lane < k ? sh.spk[lane] : 0u;
-      uint32_t iv = lane < k ? ID[pv >> 4] : 0u;
-      // bitonic sort, descending by (d2, id, image)
-      for (int size = 2; size <= 64; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-          const double od = __shfl_xor(dv, stride, 64);
-          const uint32_t op = __shfl_xor(pv, stride, 64);
-          const uint32_t oi = __shfl_xor(iv, stride, 64);
-          const bool other_greater =
-              od > dv || (od == dv && (oi > iv || (oi == iv && (op & 15u) > (pv & 15u))));
-          const bool lower = (lane & stride) == 0;
-          const bool desc = (lane & size) == 0;
-          const bool take = (lower == desc) ? other_greater : !other_greater;
-          if (take && (od != dv || oi != iv || op != pv)) {
-            dv = od; pv = op; iv = oi;
+    }
+  }
+  // misses go to the retry list, searched by k_knn_fallback
+  {
+    const bool need = valid && !resolved;
+    const unsigned long long m = ballot(need);
+    if (m) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(retry_count, (uint32_t)__popcll(m));
+      base = readlane_u(base, 0);
+      if (need) retry[base + lane_prefix(m)] = make_uint2((uint32_t)t, (uint32_t)item_idx);
+      if (lane == 0) nfall += __popcll(m & ballot(hp > 0));
+    }
+  }
+  double facc = 0.0;
+  if (valid && resolved) {
+    const double h = sqrt(tau);
+    const int kern = S->kernel;
+    {
+      // density of a fast-path query (its list was written by this lane),
+      // neighbours fetched 8 at a time
+      const u32x4* row4 = (const u32x4*)row;
+      double acc = 0.0;
+      float s_ = 0.f, c_ = 0.f;
+      const float hinv = 1.0f / (float)h;
+      for (int n0 = 0; n0 < k; n0 += 4) {
+        const u32x4 qa = row4[n0 >> 2];
+        const uint32_t pks[4] = {qa.x, qa.y, qa.z, qa.w};
+        double2 pj[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) pj[u] = XY[n0 + u < k ? (pks[u] >> 4) : (uint32_t)t];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          if (n0 + u >= k) break;
+          const uint32_t pk = pks[u];
+          const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+          const double ex = (xq + ox) - pj[u].x, ey = (yq + oy) - pj[u].y;
+          if constexpr (FP32C) {
+            const float q = sqrtf((float)(ex * ex + ey * ey)) * hinv;
+            const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
+            const float y = f - c_;
+            const float tt = s_ + y;
+            c_ = (tt - s_) - y;
+            s_ = tt;
+          } else {
+            acc += kern_F(kern, sqrt(ex * ex + ey * ey) / h);
           }
         }
       }
-      if (lane < k) {
-        nbr[t * kstride + lane] = pv;
-        sh.fterm[lane] = kern_F(kern, sqrt(dv) / h);
-      }
-      __builtin_amdgcn_wave_barrier();
-      double acc = 0.0;
-      for (int n = 0; n < k; ++n) acc += sh.fterm[n];
-      rho = S->fpref * S->mass * acc / (h * h);
+      facc = FP32C ? (double)s_ - (double)c_ : acc;
     }
-    if (lane == 0) {
-      H[t] = h;
-      RHO[t] = rho;
-      C[t] = sqrt(S->cfac * EP[t]);
+    write_result<FP32C>(S, t, xq, yq, tau, facc, EP, VPX, VPY, H, RHO, C, recp);
+  }
+  nretry += nfall;
+  for (int o = 32; o > 0; o >>= 1) {
+    nretry += __shfl_xor(nretry, o, 64);
+    nties += __shfl_xor(nties, o, 64);
+    nfail += __shfl_xor(nfail, o, 64);
+    nmiss_lo += __shfl_xor(nmiss_lo, o, 64);
+    nmiss_hi += __shfl_xor(nmiss_hi, o, 64);
+    nmiss_ann += __shfl_xor(nmiss_ann, o, 64);
+    nsplitfb += __shfl_xor(nsplitfb, o, 64);
+  }
+  if (lane == 0 && (nretry | nties | nfail | nsplitfb)) {
+    atomicAdd(&stats->retries, nretry);
+    atomicAdd(&stats->ties, nties);
+    atomicAdd(&stats->failures, nfail);
+    atomicAdd(&stats->miss_lo, nmiss_lo);
+    atomicAdd(&stats->miss_hi, nmiss_hi);
+    atomicAdd(&stats->miss_ann, nmiss_ann);
+    atomicAdd(&stats->split_fallback, nsplitfb);
+  }
+}
+
+// Exact wavefront search for the queries of the retry list (or, in
+// all-queries mode, for every particle: first pass and bitwise order).
+// Waves stride over the list, so the slow searches spread over the GPU.
+template <bool BITWISE, bool FP32C>
+__global__ __launch_bounds__(256) void k_knn_fallback(
+    const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
+    const double2* __restrict__ XY,
+    const uint8_t* __restrict__ WR, const uint32_t* __restrict__ cstart,
+    double* __restrict__ H, const double* __restrict__ EP, const double* __restrict__ VPX,
+    const double* __restrict__ VPY, const uint32_t* __restrict__ ID,
+    uint32_t* __restrict__ nbr, int kstride, double* __restrict__ RHO,
+    double* __restrict__ C, void* __restrict__ recp, KnnStats* __restrict__ stats,
+    const uint2* __restrict__ retry, const uint32_t* __restrict__ retry_count, int all_mode) {
+  __shared__ SearchShared wsh[4];
+  const int wv = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  SearchShared& sh = wsh[wv];
+  const int64_t nw = (int64_t)gridDim.x * 4;
+  const int64_t total = all_mode ? (int64_t)nitems * kItem : (int64_t)(*retry_count);
+  unsigned long long nretry = 0, nties = 0, nfail = 0;
+  for (int64_t q = (int64_t)blockIdx.x * 4 + wv; q < total; q += nw) {
+    int64_t t;
+    int item;
+    if (all_mode) {
+      item = (int)(q / kItem);
+      const int qi = (int)(q % kItem);
+      if (qi >= items[item].count) continue;
+      t = items[item].first + qi;
+    } else {
+      const uint2 e = retry[q];
+      t = e.x;
+      item = (int)e.y;
     }
-    __builtin_amdgcn_wave_barrier();
+    const SimDev* S = sims + items[item].sim;
+    const double2 pq = XY[t];
+    const double xq = pq.x, yq = pq.y, hp = H[t];
+    const double R0 = hp > 0 ? hp * 1.1 : local_radius(S, cstart, xq, yq);
+    const SearchOut so = wave_search<BITWISE, FP32C>(S, t, xq, yq, R0, XY, WR, cstart, ID,
+                                                     nbr + t * kstride, sh, nretry, nties, nfail);
+    if (lane == 0 && so.tau > 0)
+      write_result<FP32C>(S, t, xq, yq, so.tau, so.facc, EP, VPX, VPY, H, RHO, C, recp);
   }
   if (lane == 0 && (nretry | nties | nfail)) {
     atomicAdd(&stats->retries, nretry);
@@ -493,171 +920,170 @@ __global__ __launch_bounds__(256) void k_knn(
 }
 
 // ------------------------------------------------------------------------
-// Force: a group of L lanes (L = next power of two >= k) per particle, one
-// neighbour per lane, then kick / drift 2 / wrap / reflection by the
-// group's first lane.
-template <bool BITWISE, bool FP32C>
+// Force: one thread per particle, neighbours summed in list order (the
+// reference's order when the list is sorted, BITWISE), then kick, drift 2,
+// periodic wrap and reflection (sim/sph.go:327-401,122-193).
+template <bool FP32C>
 __global__ __launch_bounds__(256) void k_force(
-    const Item* __restrict__ items, int nitems, SimDev* __restrict__ sims, Positions p,
-    Fields f, const uint32_t* __restrict__ nbr, int kstride, int mode, Positions pout,
-    double* __restrict__ partials) {
+    const Item* __restrict__ items, int nitems, SimDev* __restrict__ sims,
+    const void* __restrict__ recp, Fields f, const uint32_t* __restrict__ nbr, int kstride,
+    int mode, Positions pout, double* __restrict__ partials) {
   __shared__ double red[4][kNumRed + 1];
-  __shared__ double terms[BITWISE ? 4 : 1][BITWISE ? 64 : 1][3];
   const int item_idx = xcd_item(nitems);
   const Item it = items[item_idx];
   SimDev* S = sims + it.sim;
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
+  const bool valid = (int)threadIdx.x < it.count;
+  const int64_t t = it.first + (valid ? threadIdx.x : 0);
   const int k = S->k;
-  int L = 1;
-  while (L < k) L <<= 1;
-  const int gpw = 64 / L;
-  const int g = lane / L;
-  const int n = lane % L;
   const double gamma = S->gamma, mass = S->mass;
-  const double alpha = S->alpha, beta = S->beta, eta2 = S->eta2;
   const double lx = S->lx, ly = S->ly;
   const int kern = S->kernel;
   const double dt = S->dt_half;
-  // per-lane partial reductions (leaders only contribute)
-  double r_e = 0, r_rho = 0, r_E = 0, r_px = 0, r_py = 0;
-  double r_cfl = INFINITY;
-
-  const int wbeg = wv * 64;
-  int wend = wbeg + 64;
-  if (wend > it.count) wend = it.count;
-  for (int base = wbeg; base < wend; base += gpw) {
-    const int pi = base + g;
-    const bool vp = pi < wend;
-    const int64_t t = it.first + (vp ? pi : wbeg);
-    const bool active = vp && n < k;
-    const double xi = p.x[t], yi = p.y[t];
-    const double vpxi = f.vpx[t], vpyi = f.vpy[t];
-    const double rhoi = f.rho[t], ci = f.c[t], hi = f.h[t];
-    const double A = ci * ci / (gamma * rhoi);
-    double tx = 0.0, ty = 0.0, te = 0.0;
-    if (active) {
-      const uint32_t pk = nbr[t * kstride + n];
-      const uint32_t j = pk >> 4;
-      const int img = (int)(pk & 15u);
-      const double ox = (double)(img / 3 - 1) * lx;
-      const double oy = (double)(img % 3 - 1) * ly;
-      const double xj = p.x[j], yj = p.y[j];
-      const double ex = (xi + ox) - xj, ey = (yi + oy) - yj;
-      const double d2 = ex * ex + ey * ey;
-      const double rABx = (xj - ox) - xi, rABy = (yj - oy) - yi;
-      const double vABx = f.vpx[j] - vpxi, vABy = f.vpy[j] - vpyi;
-      const double cj = f.c[j], rhoj = f.rho[j], hj = f.h[j];
-      if constexpr (!FP32C) {
-        const double d = sqrt(d2);
-        const double q = d / hi;
-        const double dR = kern_DF(kern, q);
-        const double B = cj * cj / (gamma * rhoj);
-        const double dot = vABx * rABx + vABy * rABy;
-        double piAB = 0.0;
-        if (dot < 0) {
-          const double cAB = 0.5 * (ci + cj);
-          const double rhoAB = 0.5 * (rhoi + rhoj);
-          const double hAB = 0.5 * (hi + hj);
-          const double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + eta2);
-          piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
-        }
-        tx = rABx * (piAB + A + B) * dR / d;
-        ty = rABy * (piAB + A + B) * dR / d;
-        te = dot * dR;
-      } else {
-        const float d = sqrtf((float)d2);
-        const float q = d * (1.0f / (float)hi);
-        const float dR = kern_DFf(kern, q > 1.f ? 1.f : q);
-        const float cjf = (float)cj, rjf = (float)rhoj;
-        const float B = cjf * cjf / ((float)gamma * rjf);
-        const float rx = (float)rABx, ry = (float)rABy;
-        const float vx_ = (float)vABx, vy_ = (float)vABy;
-        const float dot = vx_ * rx + vy_ * ry;
-        float piAB = 0.f;
-        if (dot < 0.f) {
-          const float cAB = 0.5f * ((float)ci + cjf);
-          const float rhoAB = 0.5f * ((float)rhoi + rjf);
-          const float hAB = 0.5f * (float)(hi + hj);
-          const float muAB = dot * hAB / ((rx * rx + ry * ry) + (float)eta2);
-          piAB = (-(float)alpha * cAB * muAB + (float)beta * muAB * muAB) / rhoAB;
-        }
-        const float sdr = (piAB + (float)A + B) * dR / d;
-        tx = (double)(rx * sdr);
-        ty = (double)(ry * sdr);
-        te = (double)(dot * dR);
+  double xi, yi, hi, ci, rhoi, A;
+  double sx = 0.0, sy = 0.0, se = 0.0;
+  const uint32_t* row = nbr + t * kstride;
+  if constexpr (!FP32C) {
+    const RecD* rec = (const RecD*)recp;
+    const RecD me = rec[t];
+    xi = me.x; yi = me.y; hi = me.h; ci = me.c; rhoi = me.rho; A = me.B;
+    const double vpxi = me.vpx, vpyi = me.vpy;
+    const double alpha = S->alpha, beta = S->beta, eta2 = S->eta2;
+    const u32x4* row4 = (const u32x4*)row;
+    for (int n4 = 0; n4 < k; n4 += 4) {
+     const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+#pragma unroll
+     for (int u = 0; u < 4; ++u) {
+      if (n4 + u >= k) break;
+      const uint32_t pk = q4[u];
+      const RecD nb = rec[pk >> 4];
+      const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+      const double ex = (xi + ox) - nb.x, ey = (yi + oy) - nb.y;
+      const double d = sqrt(ex * ex + ey * ey);
+      const double q = d / hi;
+      const double dR = kern_DF(kern, q);
+      const double rABx = (nb.x - ox) - xi, rABy = (nb.y - oy) - yi;
+      const double vABx = nb.vpx - vpxi, vABy = nb.vpy - vpyi;
+      const double dot = vABx * rABx + vABy * rABy;
+      double piAB = 0.0;
+      if (dot < 0) {
+        const double cAB = 0.5 * (ci + nb.c);
+        const double rhoAB = 0.5 * (rhoi + nb.rho);
+        const double hAB = 0.5 * (hi + nb.h);
+        const double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + eta2);
+        piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
       }
+      sx += rABx * (piAB + A + nb.B) * dR / d;
+      sy += rABy * (piAB + A + nb.B) * dR / d;
+      se += dot * dR;
+     }
     }
-    double sx, sy, se;
-    if constexpr (!BITWISE) {
-      sx = group_sum(tx, L);
-      sy = group_sum(ty, L);
-      se = group_sum(te, L);
-    } else {
-      terms[wv][lane][0] = tx;
-      terms[wv][lane][1] = ty;
-      terms[wv][lane][2] = te;
-      __builtin_amdgcn_wave_barrier();
-      sx = 0.0; sy = 0.0; se = 0.0;
-      if (n == 0) {
-        for (int m = 0; m < k; ++m) {
-          sx += terms[wv][g * L + m][0];
-          sy += terms[wv][g * L + m][1];
-          se += terms[wv][g * L + m][2];
-        }
+  } else {
+    const RecF* rec = (const RecF*)recp;
+    const RecF me = rec[t];
+    xi = me.x; yi = me.y;
+    hi = (double)me.h;
+    ci = (double)me.c;
+    rhoi = (double)me.rho;
+    A = (double)me.B;
+    const float vpxi = me.vpx, vpyi = me.vpy, cif = me.c, rhoif = me.rho, hif = me.h, Af = me.B;
+    const float hinv = __builtin_amdgcn_rcpf(hif);
+    const float alpha = (float)S->alpha, beta = (float)S->beta, eta2 = (float)S->eta2;
+    // Kahan-compensated sums
+    float fx = 0.f, fy = 0.f, fe = 0.f, cx = 0.f, cy = 0.f, ce = 0.f;
+    const u32x4* row4 = (const u32x4*)row;
+    for (int n4 = 0; n4 < k; n4 += 4) {
+     const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+#pragma unroll
+     for (int u = 0; u < 4; ++u) {
+      if (n4 + u >= k) break;
+      const uint32_t pk = q4[u];
+      const RecF nb = rec[pk >> 4];
+      const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+      const float rx = (float)((nb.x - ox) - xi), ry = (float)((nb.y - oy) - yi);
+      const float d2 = rx * rx + ry * ry;
+      const float dinv = __builtin_amdgcn_rsqf(d2);
+      const float d = d2 * dinv;
+      float q = d * hinv;
+      q = q > 1.f ? 1.f : q;
+      const float dR = kern_DFf(kern, q);
+      const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
+      const float dot = vx_ * rx + vy_ * ry;
+      float piAB = 0.f;
+      if (dot < 0.f) {
+        const float cAB = 0.5f * (cif + nb.c);
+        const float rhoAB = 0.5f * (rhoif + nb.rho);
+        const float hAB = 0.5f * (hif + nb.h);
+        const float muAB = dot * hAB * __builtin_amdgcn_rcpf((rx * rx + ry * ry) + eta2);
+        piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * __builtin_amdgcn_rcpf(rhoAB);
       }
-      __builtin_amdgcn_wave_barrier();
+      const float w = (piAB + Af + nb.B) * dR * dinv;
+      const float tx = rx * w, ty = ry * w, te = dot * dR;
+      float y_ = tx - cx, tt = fx + y_;
+      cx = (tt - fx) - y_;
+      fx = tt;
+      y_ = ty - cy; tt = fy + y_;
+      cy = (tt - fy) - y_;
+      fy = tt;
+      y_ = te - ce; tt = fe + y_;
+      ce = (tt - fe) - y_;
+      fe = tt;
+     }
     }
-    if (vp && n == 0) {
-      const double fac = mass * S->dfpref / (hi * hi * hi);
-      const double ax = sx * fac + S->gxa;
-      const double ay = sy * fac + S->gya;
-      const double edot = A * se * mass;
-      f.ax[t] = ax;
-      f.ay[t] = ay;
-      f.edot[t] = edot;
-      double vx = f.vx[t], vy = f.vy[t], e = f.e[t];
-      double x = xi, y = yi;
-      if (mode == kForceKick) {
-        // kick, sim/sph.go:122-127
-        const double adx = ax * (2 * dt), ady = ay * (2 * dt);
-        vx = vx + adx;
-        vy = vy + ady;
-        e = e + edot * 2 * dt;
-        // drift 2, :130-135
-        const double vdx = vx * dt, vdy = vy * dt;
-        x = x + vdx;
-        y = y + vdy;
-        // periodic wrap, :147-167 (an X wrap skips the Y wrap)
-        if (x < S->hor0) {
-          x += (S->hor1 - S->hor0);
-        } else if (x > S->hor1) {
-          x -= (S->hor1 - S->hor0);
-        } else if (y < S->vert0) {
-          y += (S->vert1 - S->vert0);
-        } else if (y > S->vert1) {
-          y -= (S->vert1 - S->vert0);
-        }
-        // reflection, :170-193
-        if (x < S->refl[0]) { x -= x - S->refl[0]; vx = -vx; }
-        if (x > S->refl[1]) { x -= x - S->refl[1]; vx = -vx; }
-        if (y < S->refl[2]) { y -= y - S->refl[2]; vy = -vy; }
-        if (y > S->refl[3]) { y -= y - S->refl[3]; vy = -vy; }
-        f.vx[t] = vx;
-        f.vy[t] = vy;
-        f.e[t] = e;
+    sx = (double)fx - (double)cx;
+    sy = (double)fy - (double)cy;
+    se = (double)fe - (double)ce;
+  }
+  double r_e = 0, r_rho = 0, r_E = 0, r_px = 0, r_py = 0, r_cfl = INFINITY;
+  if (valid) {
+    const double fac = mass * S->dfpref / (hi * hi * hi);
+    const double ax = sx * fac + S->gxa;
+    const double ay = sy * fac + S->gya;
+    const double edot = A * se * mass;
+    f.ax[t] = ax;
+    f.ay[t] = ay;
+    f.edot[t] = edot;
+    double vx = f.vx[t], vy = f.vy[t], e = f.e[t];
+    double x = xi, y = yi;
+    if (mode == kForceKick) {
+      // kick, sim/sph.go:122-127
+      const double adx = ax * (2 * dt), ady = ay * (2 * dt);
+      vx = vx + adx;
+      vy = vy + ady;
+      e = e + edot * 2 * dt;
+      // drift 2, :130-135
+      const double vdx = vx * dt, vdy = vy * dt;
+      x = x + vdx;
+      y = y + vdy;
+      // periodic wrap, :147-167 (an X wrap skips the Y wrap)
+      if (x < S->hor0) {
+        x += (S->hor1 - S->hor0);
+      } else if (x > S->hor1) {
+        x -= (S->hor1 - S->hor0);
+      } else if (y < S->vert0) {
+        y += (S->vert1 - S->vert0);
+      } else if (y > S->vert1) {
+        y -= (S->vert1 - S->vert0);
       }
-      pout.x[t] = x;
-      pout.y[t] = y;
-      const double sp2 = vx * vx + vy * vy;
-      const double cfl_t = (0.5 * hi) / (ci + sqrt(sp2));
-      r_cfl = cfl_t < r_cfl ? cfl_t : r_cfl;
-      r_e += e;
-      r_rho += rhoi;
-      r_E += mass * (e + 0.5 * sp2);
-      r_px += mass * vx;
-      r_py += mass * vy;
+      // reflection, :170-193
+      if (x < S->refl[0]) { x -= x - S->refl[0]; vx = -vx; }
+      if (x > S->refl[1]) { x -= x - S->refl[1]; vx = -vx; }
+      if (y < S->refl[2]) { y -= y - S->refl[2]; vy = -vy; }
+      if (y > S->refl[3]) { y -= y - S->refl[3]; vy = -vy; }
+      f.vx[t] = vx;
+      f.vy[t] = vy;
+      f.e[t] = e;
     }
+    pout.x[t] = x;
+    pout.y[t] = y;
+    const double sp2 = vx * vx + vy * vy;
+    r_cfl = (0.5 * hi) / (ci + sqrt(sp2));
+    r_e = e;
+    r_rho = rhoi;
+    r_E = mass * (e + 0.5 * sp2);
+    r_px = mass * vx;
+    r_py = mass * vy;
   }
   // block reductions in a fixed order
   r_e = group_sum(r_e, 64);
@@ -744,7 +1170,8 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   const dim3 grd(b.nitems);
   mark(0, 0);
   hipLaunchKernelGGL(k_keys, grd, blk, 0, st, b.items, b.nitems, b.sims, b.pos_cur.x,
-                     b.pos_cur.y, b.fcur.vx, b.fcur.vy, perm_mode, b.sort.keys, b.err);
+                     b.pos_cur.y, b.fcur.vx, b.fcur.vy, perm_mode, b.sort.keys, b.err,
+                     b.retry_count);
   mark(0, 1);
   mark(1, 0);
   uint32_t *skeys = nullptr, *perm = nullptr;
@@ -752,28 +1179,42 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   mark(1, 1);
   mark(2, 0);
   hipLaunchKernelGGL(k_permute, grd, blk, 0, st, b.items, b.nitems, b.sims, perm, skeys,
-                     perm_mode, b.pos_cur, b.fcur, b.pos_pass, b.fnew, b.wrap, b.cstart, b.np,
+                     perm_mode, b.pos_cur, b.fcur, b.xyp, b.fnew, b.wrap, b.cstart, b.np,
                      b.ncells);
   mark(2, 1);
   mark(3, 0);
-  const int knn_blk = 256;
-#define SPH_KNN(BW, F32)                                                                    \
-  hipLaunchKernelGGL((k_knn<BW, F32>), grd, dim3(knn_blk), 0, st, b.items, b.nitems, b.sims, \
-                     b.pos_pass.x, b.pos_pass.y, b.wrap, b.cstart, b.fnew.h, b.fnew.ep,       \
-                     b.fnew.id, b.nbr, b.kmax, b.fnew.rho, b.fnew.c, b.stats)
-  if (b.bitwise) SPH_KNN(true, false);
-  else if (b.prec == kPrecFP32C) SPH_KNN(false, true);
-  else SPH_KNN(false, false);
+  {
+    const bool fastpath = !b.bitwise && b.have_h;
+    const int fb_grid = fastpath ? 1024 : (b.nitems < 4096 ? b.nitems : 4096);
+#define SPH_KNN(F32)                                                                          \
+  hipLaunchKernelGGL((k_knn<false, F32>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,    \
+                     b.xyp, b.wrap, b.cstart, b.fnew.h, b.fnew.ep,                              \
+                     b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr, b.kmax, b.fnew.rho, b.fnew.c,    \
+                     b.rec, b.stats, b.retry, b.retry_count)
+#define SPH_FB(BW, F32)                                                                       \
+  hipLaunchKernelGGL((k_knn_fallback<BW, F32>), dim3(fb_grid), dim3(256), 0, st, b.items,     \
+                     b.nitems, b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h,                       \
+                     b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr, b.kmax, b.fnew.rho,   \
+                     b.fnew.c, b.rec, b.stats, b.retry, b.retry_count, fastpath ? 0 : 1)
+    if (fastpath) {
+      if (b.prec == kPrecFP32C) SPH_KNN(true); else SPH_KNN(false);
+    }
+    if (b.bitwise) SPH_FB(true, false);
+    else if (b.prec == kPrecFP32C) SPH_FB(false, true);
+    else SPH_FB(false, false);
 #undef SPH_KNN
+#undef SPH_FB
+  }
   mark(3, 1);
   mark(4, 0);
-#define SPH_FORCE(BW, F32)                                                                  \
-  hipLaunchKernelGGL((k_force<BW, F32>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,   \
-                     b.pos_pass, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur, b.partials)
-  if (b.bitwise) SPH_FORCE(true, false);
-  else if (b.prec == kPrecFP32C) SPH_FORCE(false, true);
-  else SPH_FORCE(false, false);
-#undef SPH_FORCE
+  if (b.prec == kPrecFP32C)
+    hipLaunchKernelGGL((k_force<true>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
+                       (const void*)b.rec, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur,
+                       b.partials);
+  else
+    hipLaunchKernelGGL((k_force<false>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
+                       (const void*)b.rec, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur,
+                       b.partials);
   mark(4, 1);
   mark(5, 0);
   hipLaunchKernelGGL(k_reduce, dim3(b.nsims), dim3(256), 0, st, b.sims, b.partials,
@@ -835,8 +1276,8 @@ int enqueue_write_field(PassBuffers& b, int field, const double* dev_in, hipStre
 // neighbour rows by ID, IDs of the neighbours, recomputed squared distances
 // (same formula as the search), sorted ascending by (d2, ID)
 __global__ void k_read_nbrs(const Item* __restrict__ items, int nitems,
-                            const SimDev* __restrict__ sims, Positions p, Fields f,
-                            const uint32_t* __restrict__ nbr, int kstride,
+                            const SimDev* __restrict__ sims, const double2* __restrict__ XY, Fields f,
+                            const uint32_t* __restrict__ nbr, int kstride, int kout,
                             uint32_t* __restrict__ ids, double* __restrict__ d2o) {
   const Item it = items[blockIdx.x];
   if ((int)threadIdx.x >= it.count) return;
@@ -844,16 +1285,15 @@ __global__ void k_read_nbrs(const Item* __restrict__ items, int nitems,
   const int64_t t = it.first + threadIdx.x;
   const int k = S->k;
   const int64_t row = (int64_t)f.id[t] + S->off;
-  uint32_t* ro = ids + row * kstride;
-  double* rd = d2o + row * kstride;
-  const double xi = p.x[t], yi = p.y[t];
+  uint32_t* ro = ids + row * kout;
+  double* rd = d2o + row * kout;
+  const double xi = XY[t].x, yi = XY[t].y;
   for (int n = 0; n < k; ++n) {
     const uint32_t pk = nbr[t * kstride + n];
     const uint32_t j = pk >> 4;
-    const int img = (int)(pk & 15u);
-    const double ox = (double)(img / 3 - 1) * S->lx;
-    const double oy = (double)(img % 3 - 1) * S->ly;
-    const double ex = (xi + ox) - p.x[j], ey = (yi + oy) - p.y[j];
+    const double ox = (double)img_ix(pk & 15u) * S->lx;
+    const double oy = (double)img_iy(pk & 15u) * S->ly;
+    const double ex = (xi + ox) - XY[j].x, ey = (yi + oy) - XY[j].y;
     const double d2 = ex * ex + ey * ey;
     const uint32_t id = f.id[j];
     int m = n;
@@ -870,7 +1310,7 @@ __global__ void k_read_nbrs(const Item* __restrict__ items, int nitems,
 int enqueue_read_neighbours(const PassBuffers& b, uint32_t* dev_ids, double* dev_d2,
                             hipStream_t st) {
   hipLaunchKernelGGL(k_read_nbrs, dim3(b.nitems), dim3(kItem), 0, st, b.items, b.nitems, b.sims,
-                     b.pos_pass, b.fcur, b.nbr, b.kmax, dev_ids, dev_d2);
+                     b.xyp, b.fcur, b.nbr, b.kmax, b.kout, dev_ids, dev_d2);
   return (int)hipGetLastError();
 }
 
