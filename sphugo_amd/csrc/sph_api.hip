@@ -278,6 +278,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
       (rc = h->alloc((char**)&pb.rec, np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) || (rc = h->alloc(&pb.partials, (int64_t)pb.nitems * kNumRed)) ||
       (rc = h->alloc(&pb.stats, 1)) || (rc = h->alloc(&pb.err, 1)) ||
       (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
+      (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, np * 16)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
       (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
       (rc = h->alloc(&pb.sort.counts, sort_counts_size(np))) || (rc = h->alloc(&pb.sort.tot, 256)) ||

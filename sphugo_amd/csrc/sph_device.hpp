@@ -155,6 +155,25 @@ __device__ inline float kern_DFf(int32_t kernel, float q) {
   return -a * a;
 }
 
+// FP64 reciprocal and reciprocal square root from the hardware estimates
+// refined by two Newton steps (explicit fma; results within about one ulp).
+// Used by the non-bitwise FP64 form, whose contract is 1e-12 relative.
+__device__ inline double rcp64(double x) {
+  double y = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, y, 1.0);
+  y = fma(y, e, y);
+  e = fma(-x, y, 1.0);
+  return fma(y, e, y);
+}
+__device__ inline double rsq64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double hx = 0.5 * x;
+  double r = fma(-hx * y, y, 0.5);
+  y = fma(y, r, y);
+  r = fma(-hx * y, y, 0.5);
+  return fma(y, r, y);
+}
+
 // ---- wave helpers (wave64) ----
 __device__ inline int lane_id() { return __lane_id(); }
 

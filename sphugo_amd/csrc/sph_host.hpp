@@ -59,6 +59,8 @@ struct PassBuffers {
   KnnStats* stats;
   uint2* retry;           // kNN retry list (np entries)
   uint32_t* retry_count;
+  uint32_t* knn_cnt;       // per query: certain | annulus << 8, or ~0 (retry)
+  uint32_t* knn_ann;       // per query: annulus candidates (kAnn entries)
   int have_h;             // every particle has an h from an earlier pass
   int* err;             // device error flag
   SortBuffers sort;

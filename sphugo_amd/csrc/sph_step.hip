@@ -144,6 +144,8 @@ constexpr int kTileCells = 256;  // coarse cells of one tile box (<= 15 x 15 use
 constexpr int kAnn = 12;       // annulus entries kept per query
 constexpr double kDelta = 0.02;  // bracket half-width around the last h
 constexpr int kSplit = 24;       // max tile candidates per query before a split
+constexpr int kSub = 8;          // sub-tiles of a wave (8 queries each)
+constexpr int kSubW = 64 / kSub;
 
 // Per-wave scratch of the exact wavefront search.
 struct SearchShared {
@@ -164,8 +166,10 @@ struct TileShared {
   int ppref[kTileCells + 1];
   int8_t ptx[kTileCells];
   int8_t pty[kTileCells];
-  double2 sxy[64];
-  uint32_t spk[64];
+  double2 sxy[65];           // staged candidates; entry 64 is a NaN sentinel
+  uint32_t spk[65];
+  double sbb[kSub][4];       // box of each sub-tile (x0, x1, y0, y1)
+  uint8_t slist[kSub][64];   // staged candidates within reach of each sub-tile
   uint32_t ann_pk[kAnn][64];
 };
 
@@ -557,7 +561,8 @@ __global__ __launch_bounds__(256) void k_knn(
     const double* __restrict__ VPY, const uint32_t* __restrict__ ID,
     uint32_t* __restrict__ nbr, int kstride, double* __restrict__ RHO,
     double* __restrict__ C, void* __restrict__ recp, KnnStats* __restrict__ stats,
-    uint2* __restrict__ retry, uint32_t* __restrict__ retry_count) {
+    uint2* __restrict__ retry, uint32_t* __restrict__ retry_count,
+    uint32_t* __restrict__ cnt, uint32_t* __restrict__ ann) {
   __shared__ TileShared wsh[4];
   const int item_idx = xcd_item(nitems);
   const Item it = items[item_idx];
@@ -585,6 +590,9 @@ __global__ __launch_bounds__(256) void k_knn(
     const double Rlo2 = Rlo * Rlo, Rhi2 = Rhi * Rhi;
     const double Rm = Rhi * (1 + 1e-12) + 1e-15 * (lx + ly);
     int c_lo = 0, c_an = 0;
+    // certain neighbours collect in cbuf and leave in 16-byte stores
+    u32x4 cbuf = {0u, 0u, 0u, 0u};
+    u32x4* row4w = (u32x4*)row;
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const uint32_t tu = (uint32_t)t;
     // Lane groups: aligned blocks of consecutive queries.  A block whose
@@ -661,7 +669,37 @@ __global__ __launch_bounds__(256) void k_knn(
         continue;
       }
       if (lane == 0) ts.ppref[npieces] = ncand;
+      {
+        // boxes of the sub-tiles (kSubW consecutive queries each)
+        double sx0 = grp ? xq - Rm : INFINITY, sx1 = grp ? xq + Rm : -INFINITY;
+        double sy0 = grp ? yq - Rm : INFINITY, sy1 = grp ? yq + Rm : -INFINITY;
+        for (int o = kSubW >> 1; o > 0; o >>= 1) {
+          sx0 = fmin(sx0, __shfl_xor(sx0, o, 64));
+          sy0 = fmin(sy0, __shfl_xor(sy0, o, 64));
+          sx1 = fmax(sx1, __shfl_xor(sx1, o, 64));
+          sy1 = fmax(sy1, __shfl_xor(sy1, o, 64));
+        }
+        if ((lane & (kSubW - 1)) == 0) {
+          ts.sbb[lane / kSubW][0] = sx0;
+          ts.sbb[lane / kSubW][1] = sx1;
+          ts.sbb[lane / kSubW][2] = sy0;
+          ts.sbb[lane / kSubW][3] = sy1;
+        }
+        if (lane == 0) {
+          ts.sxy[64] = make_double2(NAN, NAN);
+          ts.spk[64] = 0u;
+        }
+      }
       __builtin_amdgcn_wave_barrier();
+      const int my_sub = lane / kSubW;
+      double sbx0[kSub], sbx1[kSub], sby0[kSub], sby1[kSub];
+#pragma unroll
+      for (int g = 0; g < kSub; ++g) {
+        sbx0[g] = readlane_d(ts.sbb[g][0], 0);
+        sbx1[g] = readlane_d(ts.sbb[g][1], 0);
+        sby0[g] = readlane_d(ts.sbb[g][2], 0);
+        sby1[g] = readlane_d(ts.sbb[g][3], 0);
+      }
       int nsteps = 0;
       while ((1 << nsteps) < npieces) ++nsteps;
       for (int base = 0; base < ncand; base += 64) {
@@ -689,9 +727,25 @@ __global__ __launch_bounds__(256) void k_knn(
         ts.sxy[lane] = make_double2(cxv, cyv);
         ts.spk[lane] = cpk;
         const bool all_interior = ballot(interior) == ~0ull;
+        // per sub-tile, the staged candidates inside its box (in stage order)
+        int nm = 0;
+        {
+          const double px = cxv - (double)img_ix(cpk & 15u) * lx;
+          const double py = cyv - (double)img_iy(cpk & 15u) * ly;
+#pragma unroll
+          for (int g = 0; g < kSub; ++g) ts.slist[g][lane] = 64;
+#pragma unroll
+          for (int g = 0; g < kSub; ++g) {
+            const bool in = px >= sbx0[g] && px <= sbx1[g] && py >= sby0[g] && py <= sby1[g];
+            const unsigned long long m = ballot(in);
+            if (in) ts.slist[g][lane_prefix(m)] = (uint8_t)lane;
+            const int c = __popcll(m);
+            nm = c > nm ? c : nm;
+          }
+        }
         __builtin_amdgcn_wave_barrier();
-        // entries past the last candidate hold NaN and are rejected
-        const int nm = (__builtin_amdgcn_readfirstlane(ncand - base < 64 ? ncand - base : 64) + 3) & ~3;
+        nm = (__builtin_amdgcn_readfirstlane(nm) + 3) & ~3;
+        const uint8_t* mylist = ts.slist[my_sub];
 #define SPH_ACCEPT(D2, PK)                                                              \
   {                                                                                     \
     const double d2v_ = (D2);                                                           \
@@ -699,15 +753,24 @@ __global__ __launch_bounds__(256) void k_knn(
     const bool ok_ = grp && d2v_ <= Rhi2 && (pkv_ >> 4) != tu;                          \
     const bool isc_ = ok_ && d2v_ < Rlo2;                                               \
     const bool isa_ = ok_ && !(d2v_ < Rlo2);                                            \
-    if (isc_ && c_lo < k) row[c_lo] = pkv_;                                             \
+    if (isc_) {                                                                         \
+      const int sl_ = c_lo & 3;                                                         \
+      cbuf.x = sl_ == 0 ? pkv_ : cbuf.x;                                                \
+      cbuf.y = sl_ == 1 ? pkv_ : cbuf.y;                                                \
+      cbuf.z = sl_ == 2 ? pkv_ : cbuf.z;                                                \
+      cbuf.w = sl_ == 3 ? pkv_ : cbuf.w;                                                \
+      if (sl_ == 3 && c_lo < k) row4w[c_lo >> 2] = cbuf;                                \
+    }                                                                                   \
     if (isa_ && c_an < kAnn) ts.ann_pk[c_an][lane] = pkv_;                              \
     c_lo += isc_ ? 1 : 0;                                                               \
     c_an += isa_ ? 1 : 0;                                                               \
   }
         if (all_interior) {
           for (int m = 0; m < nm; m += 4) {
-            const double2 e0 = ts.sxy[m], e1 = ts.sxy[m + 1], e2 = ts.sxy[m + 2], e3 = ts.sxy[m + 3];
-            const u32x4 p4 = *(const u32x4*)&ts.spk[m];
+            const uint32_t i4 = *(const uint32_t*)(mylist + m);
+            const uint32_t i0 = i4 & 255u, i1 = (i4 >> 8) & 255u, i2 = (i4 >> 16) & 255u, i3 = i4 >> 24;
+            const double2 e0 = ts.sxy[i0], e1 = ts.sxy[i1], e2 = ts.sxy[i2], e3 = ts.sxy[i3];
+            const u32x4 p4 = {ts.spk[i0], ts.spk[i1], ts.spk[i2], ts.spk[i3]};
             double a0 = xq - e0.x, b0 = yq - e0.y;
             double a1 = xq - e1.x, b1 = yq - e1.y;
             double a2 = xq - e2.x, b2 = yq - e2.y;
@@ -721,11 +784,12 @@ __global__ __launch_bounds__(256) void k_knn(
           }
         } else {
           for (int m = 0; m < nm; ++m) {
-            const uint32_t pk = ts.spk[m];
+            const uint32_t im = mylist[m];
+            const uint32_t pk = ts.spk[im];
             const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
             const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
             const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
-            const double2 e = ts.sxy[m];
+            const double2 e = ts.sxy[im];
             const double ex = ax - e.x;
             const double ey = ay - e.y;
             SPH_ACCEPT(ex * ex + ey * ey, pk);
@@ -737,66 +801,25 @@ __global__ __launch_bounds__(256) void k_knn(
       pending &= ~gm;
       gsz = 64;
     }
-    if (ballot(fast)) {
-      // resolve: the r = k - c_lo smallest annulus entries by (d2, ID, image)
-      if (fast) {
-        if (c_lo >= k) ++nmiss_lo;
-        else if (c_an > kAnn) ++nmiss_ann;
-        else if (c_lo + c_an < k) ++nmiss_hi;
-      }
-      if (fast && c_lo < k && c_an <= kAnn && c_lo + c_an >= k) {
-        const int r = k - c_lo;
-        auto keyof = [&](uint32_t pk) {
-          return ((uint64_t)ID[pk >> 4] << 4) | img_order(pk & 15u);
-        };
-        // annulus squared distances, recomputed exactly as in the test
-        double ad[kAnn];
-        uint32_t ap[kAnn];
-#pragma unroll
-        for (int e = 0; e < kAnn; ++e) {
-          ad[e] = INFINITY;
-          ap[e] = 0;
-          if (e < c_an) {
-            const uint32_t pk = ts.ann_pk[e][lane];
-            const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
-            const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
-            const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
-            const double2 pj = XY[pk >> 4];
-            const double ex = ax - pj.x, ey = ay - pj.y;
-            ad[e] = ex * ex + ey * ey;
-            ap[e] = pk;
-          }
-        }
-        double pd = -1.0;
-        uint32_t ppk = 0;
-        for (int s = 0; s < r; ++s) {
-          double bd = INFINITY;
-          uint32_t bpk = 0;
-#pragma unroll
-          for (int e = 0; e < kAnn; ++e) {
-            const double d = ad[e];
-            const uint32_t pk = ap[e];
-            bool take = e < c_an && d >= pd && d <= bd;
-            if (take && d == pd) take = keyof(pk) > keyof(ppk);
-            if (take && d == bd) take = keyof(pk) < keyof(bpk);
-            if (take) {
-              bd = d;
-              bpk = pk;
-            }
-          }
-          row[c_lo + s] = bpk;
-          pd = bd;
-          ppk = bpk;
-        }
-        tau = pd;
-        // a tie at the boundary if an unpicked annulus entry equals tau
-#pragma unroll
-        for (int e = 0; e < kAnn; ++e)
-          if (e < c_an && ad[e] == tau && keyof(ap[e]) > keyof(ppk)) ++nties;
-        resolved = true;
-      }
-      __builtin_amdgcn_wave_barrier();
+    // hand the resolvable queries to k_knn_finish: certain neighbours are
+    // in the row, annulus candidates go to `ann`
+    if (fast) {
+      if (c_lo >= k) ++nmiss_lo;
+      else if (c_an > kAnn) ++nmiss_ann;
+      else if (c_lo + c_an < k) ++nmiss_hi;
     }
+    if (fast && c_lo < k && c_an <= kAnn && c_lo + c_an >= k) {
+      if (c_lo & 3) {
+        uint32_t* r0 = row + (c_lo & ~3);
+        r0[0] = cbuf.x;
+        if ((c_lo & 3) > 1) r0[1] = cbuf.y;
+        if ((c_lo & 3) > 2) r0[2] = cbuf.z;
+      }
+      uint32_t* ao = ann + t * kAnn;
+      for (int e = 0; e < c_an; ++e) ao[e] = ts.ann_pk[e][lane];
+      resolved = true;
+    }
+    if (valid) cnt[t] = resolved ? ((uint32_t)c_lo | ((uint32_t)c_an << 8)) : 0xFFFFFFFFu;
   }
   // misses go to the retry list, searched by k_knn_fallback
   {
@@ -809,45 +832,6 @@ __global__ __launch_bounds__(256) void k_knn(
       if (need) retry[base + lane_prefix(m)] = make_uint2((uint32_t)t, (uint32_t)item_idx);
       if (lane == 0) nfall += __popcll(m & ballot(hp > 0));
     }
-  }
-  double facc = 0.0;
-  if (valid && resolved) {
-    const double h = sqrt(tau);
-    const int kern = S->kernel;
-    {
-      // density of a fast-path query (its list was written by this lane),
-      // neighbours fetched 8 at a time
-      const u32x4* row4 = (const u32x4*)row;
-      double acc = 0.0;
-      float s_ = 0.f, c_ = 0.f;
-      const float hinv = 1.0f / (float)h;
-      for (int n0 = 0; n0 < k; n0 += 4) {
-        const u32x4 qa = row4[n0 >> 2];
-        const uint32_t pks[4] = {qa.x, qa.y, qa.z, qa.w};
-        double2 pj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) pj[u] = XY[n0 + u < k ? (pks[u] >> 4) : (uint32_t)t];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          if (n0 + u >= k) break;
-          const uint32_t pk = pks[u];
-          const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-          const double ex = (xq + ox) - pj[u].x, ey = (yq + oy) - pj[u].y;
-          if constexpr (FP32C) {
-            const float q = sqrtf((float)(ex * ex + ey * ey)) * hinv;
-            const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
-            const float y = f - c_;
-            const float tt = s_ + y;
-            c_ = (tt - s_) - y;
-            s_ = tt;
-          } else {
-            acc += kern_F(kern, sqrt(ex * ex + ey * ey) / h);
-          }
-        }
-      }
-      facc = FP32C ? (double)s_ - (double)c_ : acc;
-    }
-    write_result<FP32C>(S, t, xq, yq, tau, facc, EP, VPX, VPY, H, RHO, C, recp);
   }
   nretry += nfall;
   for (int o = 32; o > 0; o >>= 1) {
@@ -868,6 +852,143 @@ __global__ __launch_bounds__(256) void k_knn(
     atomicAdd(&stats->miss_ann, nmiss_ann);
     atomicAdd(&stats->split_fallback, nsplitfb);
   }
+}
+
+// Completes the fast-path queries: the r = k - c_lo smallest annulus
+// candidates by (d2, ID, image) fill the row, then density, sound speed and
+// the force record.  One thread per query.
+template <bool FP32C>
+__global__ __launch_bounds__(256) void k_knn_finish(
+    const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
+    const double2* __restrict__ XY, double* __restrict__ H, const double* __restrict__ EP,
+    const double* __restrict__ VPX, const double* __restrict__ VPY,
+    const uint32_t* __restrict__ ID, uint32_t* __restrict__ nbr, int kstride,
+    double* __restrict__ RHO, double* __restrict__ C, void* __restrict__ recp,
+    const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ ann,
+    KnnStats* __restrict__ stats) {
+  const Item it = items[xcd_item(nitems)];
+  const int lane = threadIdx.x & 63;
+  const bool valid = (int)threadIdx.x < it.count;
+  const int64_t t = it.first + (valid ? threadIdx.x : 0);
+  const uint32_t cp = cnt[t];
+  unsigned long long nties = 0;
+  if (valid && cp != 0xFFFFFFFFu) {
+    const SimDev* S = sims + it.sim;
+    const int k = S->k;
+    const double lx = S->lx, ly = S->ly;
+    const int c_lo = (int)(cp & 255u), c_an = (int)((cp >> 8) & 255u);
+    const int r = k - c_lo;
+    const double2 pq = XY[t];
+    const double xq = pq.x, yq = pq.y;
+    const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
+    uint32_t* row = nbr + t * kstride;
+    // annulus squared distances, recomputed exactly as in the walk
+    const uint32_t* ai = ann + t * kAnn;
+    uint32_t ap[kAnn];
+    double ad[kAnn];
+#pragma unroll
+    for (int e = 0; e < kAnn; ++e) ap[e] = e < c_an ? ai[e] : (uint32_t)(t << 4) | 5u;
+#pragma unroll
+    for (int e = 0; e < kAnn; ++e) {
+      const uint32_t pk = ap[e];
+      const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
+      const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
+      const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
+      const double2 pj = XY[pk >> 4];
+      const double ex = ax - pj.x, ey = ay - pj.y;
+      ad[e] = e < c_an ? ex * ex + ey * ey : INFINITY;
+    }
+    // r smallest by (d2, slot)
+    double pd = -1.0;
+    int ps = -1;
+    for (int s_ = 0; s_ < r; ++s_) {
+      double bd = INFINITY;
+      int bs = kAnn;
+      uint32_t bpk = 0;
+#pragma unroll
+      for (int e = 0; e < kAnn; ++e) {
+        const double d = ad[e];
+        const bool after = d > pd || (d == pd && e > ps);
+        const bool before = d < bd;
+        if (after && before) {
+          bd = d;
+          bs = e;
+          bpk = ap[e];
+        }
+      }
+      row[c_lo + s_] = bpk;
+      pd = bd;
+      ps = bs;
+    }
+    double tau = pd;
+    // an annulus entry left out with d2 == tau: resolve the tie exactly
+    bool tie = false;
+#pragma unroll
+    for (int e = 0; e < kAnn; ++e) tie = tie || (e > ps && ad[e] == tau);
+    if (tie) {
+      ++nties;
+      int below = 0;
+#pragma unroll
+      for (int e = 0; e < kAnn; ++e) below += ad[e] < tau ? 1 : 0;
+      // the k - c_lo - below slots after the entries below tau take the
+      // tied entries with the smallest (ID, image order)
+      uint64_t prev = 0;
+      bool first = true;
+      for (int s_ = c_lo + below; s_ < k; ++s_) {
+        uint64_t best = ~0ull;
+        uint32_t bpk = 0;
+        for (int e = 0; e < c_an; ++e) {
+          if (ad[e] != tau) continue;
+          const uint64_t key = ((uint64_t)ID[ap[e] >> 4] << 4) | img_order(ap[e] & 15u);
+          if ((first || key > prev) && key < best) {
+            best = key;
+            bpk = ap[e];
+          }
+        }
+        row[s_] = bpk;
+        prev = best;
+        first = false;
+      }
+    }
+    // density: the row holds the k neighbours; fetch them 8 at a time
+    const double h = sqrt(tau);
+    const int kern = S->kernel;
+    const u32x4* row4 = (const u32x4*)row;
+    double acc = 0.0;
+    float s_f = 0.f, c_f = 0.f;
+    const float hinv = 1.0f / (float)h;
+    const double hinv64 = 1.0 / h;
+    for (int n0 = 0; n0 < k; n0 += 8) {
+      const u32x4 qa = row4[n0 >> 2];
+      const u32x4 qb = n0 + 4 < k ? row4[(n0 >> 2) + 1] : qa;
+      const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+      double2 pj[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) pj[u] = XY[n0 + u < k ? (pks[u] >> 4) : (uint32_t)t];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        if (n0 + u >= k) break;
+        const uint32_t pk = pks[u];
+        const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+        const double ex = (xq + ox) - pj[u].x, ey = (yq + oy) - pj[u].y;
+        if constexpr (FP32C) {
+          const float q = sqrtf((float)(ex * ex + ey * ey)) * hinv;
+          const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
+          const float y = f - c_f;
+          const float tt = s_f + y;
+          c_f = (tt - s_f) - y;
+          s_f = tt;
+        } else {
+          const double d2 = ex * ex + ey * ey;
+          acc += kern_F(kern, (d2 * rsq64(d2)) * hinv64);
+        }
+      }
+    }
+    const double facc = FP32C ? (double)s_f - (double)c_f : acc;
+    write_result<FP32C>(S, t, xq, yq, tau, facc, EP, VPX, VPY, H, RHO, C, recp);
+  }
+  for (int o = 32; o > 0; o >>= 1) nties += __shfl_xor(nties, o, 64);
+  if (lane == 0 && nties) atomicAdd(&stats->ties, nties);
 }
 
 // Exact wavefront search for the queries of the retry list (or, in
@@ -923,7 +1044,7 @@ __global__ __launch_bounds__(256) void k_knn_fallback(
 // Force: one thread per particle, neighbours summed in list order (the
 // reference's order when the list is sorted, BITWISE), then kick, drift 2,
 // periodic wrap and reflection (sim/sph.go:327-401,122-193).
-template <bool FP32C>
+template <bool FP32C, bool EXACT>
 __global__ __launch_bounds__(256) void k_force(
     const Item* __restrict__ items, int nitems, SimDev* __restrict__ sims,
     const void* __restrict__ recp, Fields f, const uint32_t* __restrict__ nbr, int kstride,
@@ -937,47 +1058,71 @@ __global__ __launch_bounds__(256) void k_force(
   const bool valid = (int)threadIdx.x < it.count;
   const int64_t t = it.first + (valid ? threadIdx.x : 0);
   const int k = S->k;
-  const double gamma = S->gamma, mass = S->mass;
+  const double mass = S->mass;
   const double lx = S->lx, ly = S->ly;
   const int kern = S->kernel;
   const double dt = S->dt_half;
   double xi, yi, hi, ci, rhoi, A;
   double sx = 0.0, sy = 0.0, se = 0.0;
   const uint32_t* row = nbr + t * kstride;
+  const u32x4* row4 = (const u32x4*)row;
   if constexpr (!FP32C) {
     const RecD* rec = (const RecD*)recp;
     const RecD me = rec[t];
     xi = me.x; yi = me.y; hi = me.h; ci = me.c; rhoi = me.rho; A = me.B;
     const double vpxi = me.vpx, vpyi = me.vpy;
     const double alpha = S->alpha, beta = S->beta, eta2 = S->eta2;
-    const u32x4* row4 = (const u32x4*)row;
+    const double hinv = EXACT ? 0.0 : 1.0 / hi;
     for (int n4 = 0; n4 < k; n4 += 4) {
-     const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+      const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+      const uint32_t pks[4] = {q4.x, q4.y, q4.z, q4.w};
+      RecD nbs[4];
 #pragma unroll
-     for (int u = 0; u < 4; ++u) {
-      if (n4 + u >= k) break;
-      const uint32_t pk = q4[u];
-      const RecD nb = rec[pk >> 4];
-      const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-      const double ex = (xi + ox) - nb.x, ey = (yi + oy) - nb.y;
-      const double d = sqrt(ex * ex + ey * ey);
-      const double q = d / hi;
-      const double dR = kern_DF(kern, q);
-      const double rABx = (nb.x - ox) - xi, rABy = (nb.y - oy) - yi;
-      const double vABx = nb.vpx - vpxi, vABy = nb.vpy - vpyi;
-      const double dot = vABx * rABx + vABy * rABy;
-      double piAB = 0.0;
-      if (dot < 0) {
-        const double cAB = 0.5 * (ci + nb.c);
-        const double rhoAB = 0.5 * (rhoi + nb.rho);
-        const double hAB = 0.5 * (hi + nb.h);
-        const double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + eta2);
-        piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
+      for (int u = 0; u < 4; ++u) nbs[u] = rec[n4 + u < k ? (pks[u] >> 4) : (uint32_t)t];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (n4 + u >= k) break;
+        const uint32_t pk = pks[u];
+        const RecD& nb = nbs[u];
+        const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+        const double ex = (xi + ox) - nb.x, ey = (yi + oy) - nb.y;
+        const double d2 = ex * ex + ey * ey;
+        const double rABx = (nb.x - ox) - xi, rABy = (nb.y - oy) - yi;
+        const double vABx = nb.vpx - vpxi, vABy = nb.vpy - vpyi;
+        const double dot = vABx * rABx + vABy * rABy;
+        if constexpr (EXACT) {
+          // the reference's arithmetic, operation for operation
+          const double d = sqrt(d2);
+          const double q = d / hi;
+          const double dR = kern_DF(kern, q);
+          double piAB = 0.0;
+          if (dot < 0) {
+            const double cAB = 0.5 * (ci + nb.c);
+            const double rhoAB = 0.5 * (rhoi + nb.rho);
+            const double hAB = 0.5 * (hi + nb.h);
+            const double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + eta2);
+            piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
+          }
+          sx += rABx * (piAB + A + nb.B) * dR / d;
+          sy += rABy * (piAB + A + nb.B) * dR / d;
+          se += dot * dR;
+        } else {
+          const double rinv = rsq64(d2);
+          const double q = (d2 * rinv) * hinv;
+          const double dR = kern_DF(kern, q);
+          double piAB = 0.0;
+          if (dot < 0) {
+            const double cAB = 0.5 * (ci + nb.c);
+            const double hAB = 0.5 * (hi + nb.h);
+            const double muAB = dot * hAB * rcp64((rABx * rABx + rABy * rABy) + eta2);
+            piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * rcp64(0.5 * (rhoi + nb.rho));
+          }
+          const double w = (piAB + A + nb.B) * dR * rinv;
+          sx += rABx * w;
+          sy += rABy * w;
+          se += dot * dR;
+        }
       }
-      sx += rABx * (piAB + A + nb.B) * dR / d;
-      sy += rABy * (piAB + A + nb.B) * dR / d;
-      se += dot * dR;
-     }
     }
   } else {
     const RecF* rec = (const RecF*)recp;
@@ -992,44 +1137,46 @@ __global__ __launch_bounds__(256) void k_force(
     const float alpha = (float)S->alpha, beta = (float)S->beta, eta2 = (float)S->eta2;
     // Kahan-compensated sums
     float fx = 0.f, fy = 0.f, fe = 0.f, cx = 0.f, cy = 0.f, ce = 0.f;
-    const u32x4* row4 = (const u32x4*)row;
     for (int n4 = 0; n4 < k; n4 += 4) {
-     const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+      const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+      const uint32_t pks[4] = {q4.x, q4.y, q4.z, q4.w};
+      RecF nbs[4];
 #pragma unroll
-     for (int u = 0; u < 4; ++u) {
-      if (n4 + u >= k) break;
-      const uint32_t pk = q4[u];
-      const RecF nb = rec[pk >> 4];
-      const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-      const float rx = (float)((nb.x - ox) - xi), ry = (float)((nb.y - oy) - yi);
-      const float d2 = rx * rx + ry * ry;
-      const float dinv = __builtin_amdgcn_rsqf(d2);
-      const float d = d2 * dinv;
-      float q = d * hinv;
-      q = q > 1.f ? 1.f : q;
-      const float dR = kern_DFf(kern, q);
-      const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
-      const float dot = vx_ * rx + vy_ * ry;
-      float piAB = 0.f;
-      if (dot < 0.f) {
-        const float cAB = 0.5f * (cif + nb.c);
-        const float rhoAB = 0.5f * (rhoif + nb.rho);
-        const float hAB = 0.5f * (hif + nb.h);
-        const float muAB = dot * hAB * __builtin_amdgcn_rcpf((rx * rx + ry * ry) + eta2);
-        piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * __builtin_amdgcn_rcpf(rhoAB);
+      for (int u = 0; u < 4; ++u) nbs[u] = rec[n4 + u < k ? (pks[u] >> 4) : (uint32_t)t];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (n4 + u >= k) break;
+        const uint32_t pk = pks[u];
+        const RecF& nb = nbs[u];
+        const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+        const float rx = (float)((nb.x - ox) - xi), ry = (float)((nb.y - oy) - yi);
+        const float d2 = rx * rx + ry * ry;
+        const float dinv = __builtin_amdgcn_rsqf(d2);
+        float q = (d2 * dinv) * hinv;
+        q = q > 1.f ? 1.f : q;
+        const float dR = kern_DFf(kern, q);
+        const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
+        const float dot = vx_ * rx + vy_ * ry;
+        float piAB = 0.f;
+        if (dot < 0.f) {
+          const float cAB = 0.5f * (cif + nb.c);
+          const float rhoAB = 0.5f * (rhoif + nb.rho);
+          const float hAB = 0.5f * (hif + nb.h);
+          const float muAB = dot * hAB * __builtin_amdgcn_rcpf((rx * rx + ry * ry) + eta2);
+          piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * __builtin_amdgcn_rcpf(rhoAB);
+        }
+        const float w = (piAB + Af + nb.B) * dR * dinv;
+        const float tx = rx * w, ty = ry * w, te = dot * dR;
+        float y_ = tx - cx, tt = fx + y_;
+        cx = (tt - fx) - y_;
+        fx = tt;
+        y_ = ty - cy; tt = fy + y_;
+        cy = (tt - fy) - y_;
+        fy = tt;
+        y_ = te - ce; tt = fe + y_;
+        ce = (tt - fe) - y_;
+        fe = tt;
       }
-      const float w = (piAB + Af + nb.B) * dR * dinv;
-      const float tx = rx * w, ty = ry * w, te = dot * dR;
-      float y_ = tx - cx, tt = fx + y_;
-      cx = (tt - fx) - y_;
-      fx = tt;
-      y_ = ty - cy; tt = fy + y_;
-      cy = (tt - fy) - y_;
-      fy = tt;
-      y_ = te - ce; tt = fe + y_;
-      ce = (tt - fe) - y_;
-      fe = tt;
-     }
     }
     sx = (double)fx - (double)cx;
     sy = (double)fy - (double)cy;
@@ -1190,7 +1337,7 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   hipLaunchKernelGGL((k_knn<false, F32>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,    \
                      b.xyp, b.wrap, b.cstart, b.fnew.h, b.fnew.ep,                              \
                      b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr, b.kmax, b.fnew.rho, b.fnew.c,    \
-                     b.rec, b.stats, b.retry, b.retry_count)
+                     b.rec, b.stats, b.retry, b.retry_count, b.knn_cnt, b.knn_ann)
 #define SPH_FB(BW, F32)                                                                       \
   hipLaunchKernelGGL((k_knn_fallback<BW, F32>), dim3(fb_grid), dim3(256), 0, st, b.items,     \
                      b.nitems, b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h,                       \
@@ -1198,6 +1345,14 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
                      b.fnew.c, b.rec, b.stats, b.retry, b.retry_count, fastpath ? 0 : 1)
     if (fastpath) {
       if (b.prec == kPrecFP32C) SPH_KNN(true); else SPH_KNN(false);
+      if (b.prec == kPrecFP32C)
+        hipLaunchKernelGGL((k_knn_finish<true>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
+                           b.xyp, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
+                           b.kmax, b.fnew.rho, b.fnew.c, b.rec, b.knn_cnt, b.knn_ann, b.stats);
+      else
+        hipLaunchKernelGGL((k_knn_finish<false>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
+                           b.xyp, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
+                           b.kmax, b.fnew.rho, b.fnew.c, b.rec, b.knn_cnt, b.knn_ann, b.stats);
     }
     if (b.bitwise) SPH_FB(true, false);
     else if (b.prec == kPrecFP32C) SPH_FB(false, true);
@@ -1207,14 +1362,13 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   }
   mark(3, 1);
   mark(4, 0);
-  if (b.prec == kPrecFP32C)
-    hipLaunchKernelGGL((k_force<true>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
-                       (const void*)b.rec, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur,
-                       b.partials);
-  else
-    hipLaunchKernelGGL((k_force<false>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
-                       (const void*)b.rec, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur,
-                       b.partials);
+#define SPH_FORCE(F32, EX)                                                                     \
+  hipLaunchKernelGGL((k_force<F32, EX>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,     \
+                     (const void*)b.rec, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur, b.partials)
+  if (b.prec == kPrecFP32C) SPH_FORCE(true, false);
+  else if (b.bitwise) SPH_FORCE(false, true);
+  else SPH_FORCE(false, false);
+#undef SPH_FORCE
   mark(4, 1);
   mark(5, 0);
   hipLaunchKernelGGL(k_reduce, dim3(b.nsims), dim3(256), 0, st, b.sims, b.partials,

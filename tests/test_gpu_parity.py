@@ -163,9 +163,11 @@ def test_totals_and_quirks():
     o.step(3)
     assert np.isclose(g.total_energy(), o.total_energy(), rtol=1e-12)
     assert np.isclose(g.total_density(), o.total_density(), rtol=1e-10)
-    # TotalMomentum is the speed of the particle stored last: some particle
-    sp = np.hypot(g.get("vx"), g.get("vy"))
-    assert np.isclose(sp, g.total_momentum(), rtol=0, atol=0).any()
+    # TotalMomentum is the speed of the particle stored last: some particle's
+    # Vel.Norm() = sqrt(v.Dot(v)) (sim/linear-algebra.go:53-55)
+    vx, vy = g.get("vx"), g.get("vy")
+    sp = np.sqrt(vx * vx + vy * vy)
+    assert (sp == g.total_momentum()).any()
 
 
 def test_wrap_quirk_on_device():
