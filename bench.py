@@ -46,6 +46,10 @@ def algo_bytes(kernel: str, k: int) -> int:
 
 
 def make_state(args):
+    if args.workload == "batch":
+        sts = ics.batch(args.batch_count or 512, seed=4, k=32)
+        return {"name": f"batch-{len(sts)}", "batch": sts, "k": 32,
+                "n": int(sum(st["n"] for st in sts))}
     if args.workload == "clustered":
         return ics.clustered(args.n or 4194304, seed=3, k=64)
     if args.workload == "blast":
@@ -61,8 +65,21 @@ def params_for(st, precision):
                             visc_alpha=1.0, visc_beta=2.0, visc_eta2=0.01)
 
 
-def run_variant(st, precision, steps, warmup, timing_steps=3):
-    sim = L.Sim(params_for(st, precision), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+def make_sim(st, precision, graph=False, tune=False):
+    if "batch" in st:
+        sim = L.Batch([params_for(b, precision) for b in st["batch"]], st["batch"])
+    else:
+        sim = L.Sim(params_for(st, precision), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    tuned = None
+    if tune:
+        tuned = L.autotune(sim)
+    if graph:
+        L.set_graph(sim, True)
+    return sim, tuned
+
+
+def run_variant(st, precision, steps, warmup, timing_steps=3, graph=False, tune=False):
+    sim, tuned = make_sim(st, precision, graph, tune)
     sim.step(warmup)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -75,6 +92,8 @@ def run_variant(st, precision, steps, warmup, timing_steps=3):
     sim.step(timing_steps)
     ms, cnt = sim.kernel_times()
     sim.set_timing(False)
+    if graph:
+        L.set_graph(sim, True)
     retries, ties = sim.knn_stats()
     counters = sim.knn_counters()
     red = sim.reductions()
@@ -87,6 +106,8 @@ def run_variant(st, precision, steps, warmup, timing_steps=3):
         "knn_retries": retries,
         "knn_counters": counters,
         "dt_half_last": red[5],
+        "autotune": None if tuned is None else {"pass_ms": round(tuned[0], 4),
+                                                "cells_per_particle": tuned[1]},
     }
 
 
@@ -125,6 +146,9 @@ def main():
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--precision", default="both", choices=["both", "fp32c", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--graph", action="store_true", help="replay steps from a hipGraph")
+    ap.add_argument("--tune", action="store_true", help="autotune the grid before stepping")
+    ap.add_argument("--batch-count", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -134,10 +158,12 @@ def main():
     k = st["k"]
     res = {}
     if args.precision in ("both", "fp32c"):
-        res["fp32c"] = run_variant(st, L.PREC_FP32C, args.steps, args.warmup)
+        res["fp32c"] = run_variant(st, L.PREC_FP32C, args.steps, args.warmup,
+                                   graph=args.graph, tune=args.tune)
         print(f"# fp32c: {res['fp32c']}", file=sys.stderr, flush=True)
     if args.precision in ("both", "fp64"):
-        res["fp64"] = run_variant(st, L.PREC_FP64, args.steps, args.warmup)
+        res["fp64"] = run_variant(st, L.PREC_FP64, args.steps, args.warmup,
+                                  graph=args.graph, tune=args.tune)
         print(f"# fp64: {res['fp64']}", file=sys.stderr, flush=True)
     head = res.get("fp32c") or res["fp64"]
     # dominant kernel and its roofline

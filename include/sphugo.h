@@ -168,6 +168,31 @@ int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties);
  * annulus over capacity), tiles sent to the wavefront search. */
 int sph_knn_counters(sph_sim* s, int64_t out[7]);
 
+/* ---- batches of independent simulations (no reference counterpart) ----
+ * One handle holding `count` simulations; every kernel of a step covers all
+ * of them in one launch over (simulation, tile) work items.  Precision and
+ * the bitwise flag are shared; k, kernel, physics and box are per
+ * simulation.  x[i], y[i], vx[i], vy[i], e[i] hold simulation i's n[i]
+ * particles (vx, vy, e or their entries may be NULL).  sph_step and the
+ * whole-handle readbacks work on the concatenation of all simulations. */
+int sph_batch_create(int32_t count, const sph_params* params, const int64_t* n,
+                     const double* const* x, const double* const* y,
+                     const double* const* vx, const double* const* vy,
+                     const double* const* e, sph_sim** out);
+int32_t sph_batch_count(const sph_sim* s);
+int64_t sph_batch_num_particles(const sph_sim* s, int32_t sim);
+int sph_batch_read_field(sph_sim* s, int32_t sim, int32_t field, double* out);
+int sph_batch_write_field(sph_sim* s, int32_t sim, int32_t field, const double* in);
+int sph_batch_reductions(sph_sim* s, int32_t sim, double out[6]);
+
+/* Replay regular steps from a captured hipGraph (1) or launch them one by
+ * one (0).  Results are identical either way. */
+int sph_set_graph(sph_sim* s, int32_t on);
+/* Time force passes over the candidate Morton grid resolutions (cells per
+ * particle 0.5 ... 16) on the current state and keep the fastest.  Only
+ * before the first step (SPH_ESTATE otherwise).  Either output may be NULL. */
+int sph_autotune(sph_sim* s, double* best_ms, double* best_cells_per_particle);
+
 const char* sph_last_error(void);
 const char* sph_version(void);
 

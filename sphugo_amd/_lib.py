@@ -78,6 +78,17 @@ SIGNATURES = {
     "sph_knn_stats": (ctypes.c_int, [_VP, _I64, _I64]),
     "sph_set_timing": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "sph_knn_counters": (ctypes.c_int, [_VP, _I64]),
+    "sph_batch_create": (ctypes.c_int, [ctypes.c_int32, ctypes.POINTER(Params), _I64,
+                                        ctypes.POINTER(_D), ctypes.POINTER(_D), ctypes.POINTER(_D),
+                                        ctypes.POINTER(_D), ctypes.POINTER(_D),
+                                        ctypes.POINTER(_VP)]),
+    "sph_batch_count": (ctypes.c_int32, [_VP]),
+    "sph_batch_num_particles": (ctypes.c_int64, [_VP, ctypes.c_int32]),
+    "sph_batch_read_field": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, _D]),
+    "sph_batch_write_field": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, _D]),
+    "sph_batch_reductions": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
+    "sph_set_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
+    "sph_autotune": (ctypes.c_int, [_VP, _D, _D]),
     "sph_last_error": (ctypes.c_char_p, []),
     "sph_version": (ctypes.c_char_p, []),
 }
@@ -237,3 +248,62 @@ class Sim:
         t = ctypes.c_int64()
         check(lib().sph_knn_stats(self._h, ctypes.byref(r), ctypes.byref(t)))
         return r.value, t.value
+
+
+class Batch(Sim):
+    """Several independent simulations stepped together (sph_batch_create)."""
+
+    def __init__(self, params_list, states):
+        count = len(states)
+        assert count == len(params_list) and count > 0
+        P = (Params * count)(*params_list)
+        ns = np.array([len(st["x"]) for st in states], np.int64)
+        keep = []
+
+        def ptrs(name):
+            arr = (_D * count)()
+            for i, st in enumerate(states):
+                a = st.get(name)
+                if a is None:
+                    arr[i] = None
+                else:
+                    a = np.ascontiguousarray(a, np.float64)
+                    keep.append(a)
+                    arr[i] = dptr(a)
+            return arr
+
+        xs, ys, vxs, vys, es = (ptrs(n) for n in ("x", "y", "vx", "vy", "e"))
+        h = _VP()
+        check(lib().sph_batch_create(count, P, ns.ctypes.data_as(_I64), xs, ys, vxs, vys, es,
+                                     ctypes.byref(h)))
+        self._h = h
+        self.count = count
+        self.sizes = ns
+        self.n = int(ns.sum())
+        self.k = max(p.k for p in params_list)
+        self.params = params_list[0]
+
+    def get_sim(self, i, name):
+        out = np.empty(int(self.sizes[i]))
+        check(lib().sph_batch_read_field(self._h, i, FIELD[name], dptr(out)))
+        return out
+
+    def set_sim(self, i, name, values):
+        v = np.ascontiguousarray(values, np.float64)
+        check(lib().sph_batch_write_field(self._h, i, FIELD[name], dptr(v)))
+
+    def reductions_sim(self, i):
+        out = np.empty(6)
+        check(lib().sph_batch_reductions(self._h, i, dptr(out)))
+        return out
+
+
+def set_graph(sim, on):
+    check(lib().sph_set_graph(sim._h, 1 if on else 0))
+
+
+def autotune(sim):
+    ms = ctypes.c_double()
+    cpp = ctypes.c_double()
+    check(lib().sph_autotune(sim._h, ctypes.byref(ms), ctypes.byref(cpp)))
+    return ms.value, cpp.value

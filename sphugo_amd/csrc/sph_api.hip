@@ -63,6 +63,11 @@ struct sph_sim {
   uint32_t* d_nids = nullptr;      // neighbour readback (lazy)
   double* d_nd2 = nullptr;
   KnnStats last_stats = {};
+  // graph replay of a regular step, one per field-buffer parity
+  bool graph_on = false;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double grid_cpp = 4.0;        // cells per particle of the Morton grid
+  int64_t cell_capacity = 0;    // cstart entries allocated
 
   template <class T>
   int alloc(T** p, int64_t count) {
@@ -74,7 +79,15 @@ struct sph_sim {
     *p = (T*)q;
     return 0;
   }
+  void drop_graphs() {
+    for (auto& g : gexec)
+      if (g) {
+        hipGraphExecDestroy(g);
+        g = nullptr;
+      }
+  }
   ~sph_sim() {
+    drop_graphs();
     for (void* q : allocs) hipFree(q);
     for (auto& e : ev)
       if (e) hipEventDestroy(e);
@@ -183,6 +196,36 @@ static void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
   d.r_init = 1.2 * std::sqrt(p->k * d.lx * d.ly / (M_PI * (double)n));
 }
 
+static void set_grid(SimDev& d, int lg) {
+  const int g = 1 << lg;
+  d.lg = lg;
+  d.gx = g;
+  d.gy = g;
+  d.ivx = g / d.lx;
+  d.ivy = g / d.ly;
+  d.sx = d.lx / g;
+  d.sy = d.ly / g;
+}
+
+// Cell offsets and key width of all simulations for `cpp` cells per particle.
+static int64_t layout_cells(sph_sim* h, double cpp) {
+  int64_t coff = 0;
+  for (size_t s = 0; s < h->hsims.size(); ++s) {
+    set_grid(h->hsims[s], grid_log2(h->sim_n[s], cpp));
+    h->hsims[s].coff = coff;
+    coff += (int64_t)h->hsims[s].gx * h->hsims[s].gy;
+  }
+  int bits = 1;
+  while ((1LL << bits) <= coff) ++bits;
+  h->pb.ncells = coff;
+  h->pb.key_bits = bits;
+  h->grid_cpp = cpp;
+  return coff;
+}
+
+static const double kTuneCpp[] = {0.5, 1.0, 2.0, 4.0, 8.0, 16.0};
+static int fetch_sims(sph_sim* s);
+
 // Build a handle holding `count` simulations.
 static int create_batch(int count, const sph_params* const* ps, const int64_t* ns,
                         const double* const* xs, const double* const* ys,
@@ -225,10 +268,8 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   for (int s = 0; s < count; ++s) {
     SimDev d;
     const int lg = grid_log2(ns[s], 4.0);
-    const int g = 1 << lg;
     fill_simdev(d, ps[s], ns[s], lg);
     d.off = off;
-    d.coff = coff;
     h->hsims.push_back(d);
     h->sim_n.push_back(ns[s]);
     red_first[s] = (int)h->hitems.size();
@@ -241,11 +282,15 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
     }
     red_count[s] = (int)h->hitems.size() - red_first[s];
     off += ns[s];
-    coff += (int64_t)g * g;
+  }
+  // cell storage for the finest grid the autotuner may choose
+  for (double cpp : kTuneCpp) {
+    const int64_t c = layout_cells(h, cpp);
+    coff = c > coff ? c : coff;
   }
   if (coff >= (1LL << 32) - 1) return bail(fail(SPH_EINVAL, "too many cells"));
-  int bits = 1;
-  while ((1LL << bits) <= coff) ++bits;
+  h->cell_capacity = coff + 1;
+  layout_cells(h, 4.0);
   PassBuffers& pb = h->pb;
   pb.nsims = count;
   pb.nitems = (int)h->hitems.size();
@@ -254,8 +299,6 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   pb.prec = h->prec;
   pb.bitwise = h->bitwise;
   pb.np = np;
-  pb.ncells = coff;
-  pb.key_bits = bits;
   // device memory
   Item* d_items = nullptr;
   SimDev* d_sims = nullptr;
@@ -273,7 +316,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   if ((rc = h->alloc(&h->P[0].x, np)) || (rc = h->alloc(&h->P[0].y, np)) ||
       (rc = h->alloc(&pb.xyp, np)))
     return bail(rc);
-  if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, coff + 1)) ||
+  if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, h->cell_capacity)) ||
       (rc = h->alloc(&pb.nbr, np * pb.kmax)) ||
       (rc = h->alloc((char**)&pb.rec, np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) || (rc = h->alloc(&pb.partials, (int64_t)pb.nitems * kNumRed)) ||
       (rc = h->alloc(&pb.stats, 1)) || (rc = h->alloc(&pb.err, 1)) ||
@@ -389,16 +432,105 @@ static int check_device_flags(sph_sim* s) {
   return 0;
 }
 
+// One regular step from a captured graph (captured on first use for the
+// current field-buffer parity).  The kernels read every per-step value
+// (half step, counters) from device memory, so replays need no host input.
+static int graph_step(sph_sim* s) {
+  const int g = s->cur;
+  if (!s->gexec[g]) {
+    s->bind();
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(s->st, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue_pass(s->pb, kPermPredict, kForceKick, s->st, nullptr);
+    hipError_t e = hipStreamEndCapture(s->st, &graph);
+    if (rc || e != hipSuccess) {
+      if (graph) hipGraphDestroy(graph);
+      return fail(SPH_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&s->gexec[g], graph, nullptr, nullptr, 0);
+    hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(SPH_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
+  }
+  HIPCHK(hipGraphLaunch(s->gexec[g], s->st));
+  s->cur ^= 1;
+  s->bind();
+  return 0;
+}
+
 extern "C" int sph_step_async(sph_sim* s, int32_t nsteps) {
   if (!s) return fail(SPH_EINVAL, "null handle");
   if (nsteps < 0) return fail(SPH_EINVAL, "negative step count");
   for (int32_t i = 0; i < nsteps; ++i) {
     int rc;
     if (s->step == 0 && (rc = run_pass(s, kPermInit, kForceNoKick))) return rc;
-    if ((rc = run_pass(s, kPermPredict, kForceKick))) return rc;
+    if (s->graph_on && !s->timing && s->pb.have_h) rc = graph_step(s);
+    else rc = run_pass(s, kPermPredict, kForceKick);
+    if (rc) return rc;
     s->step += 1;
   }
   return 0;
+}
+
+extern "C" int sph_set_graph(sph_sim* s, int32_t on) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->drop_graphs();
+  s->graph_on = on != 0;
+  return 0;
+}
+
+// Autotuner: times force passes over the candidate grid resolutions on the
+// current state (only before the first step, where they cannot change the
+// trajectory) and keeps the fastest.
+extern "C" int sph_autotune(sph_sim* s, double* best_ms, double* best_cpp) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  if (s->step != 0) return fail(SPH_ESTATE, "autotune runs before the first step");
+  HIPCHK(hipStreamSynchronize(s->st));
+  s->drop_graphs();
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  double best = 1e30, bcpp = s->grid_cpp;
+  const bool timing = s->timing;
+  s->timing = false;
+  int rc = 0;
+  for (double cpp : kTuneCpp) {
+    // keep the device-updated fields (half step, CFL minimum, totals)
+    if ((rc = fetch_sims(s))) break;
+    layout_cells(s, cpp);
+    if (hipMemcpyAsync(s->pb.sims, s->hsims.data(), s->hsims.size() * sizeof(SimDev),
+                       hipMemcpyHostToDevice, s->st) != hipSuccess) {
+      rc = fail(SPH_EDEVICE, "upload failed");
+      break;
+    }
+    // a pass to set h, then the timed pass
+    if ((rc = run_pass(s, kPermInit, kForceNoKick))) break;
+    hipEventRecord(e0, s->st);
+    if ((rc = run_pass(s, kPermInit, kForceNoKick))) break;
+    hipEventRecord(e1, s->st);
+    if (hipEventSynchronize(e1) != hipSuccess) {
+      rc = fail(SPH_EDEVICE, "autotune pass failed");
+      break;
+    }
+    float ms = 0;
+    hipEventElapsedTime(&ms, e0, e1);
+    if (ms < best) {
+      best = ms;
+      bcpp = cpp;
+    }
+  }
+  s->timing = timing;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  if ((rc = fetch_sims(s))) return rc;
+  layout_cells(s, bcpp);
+  HIPCHK(hipMemcpyAsync(s->pb.sims, s->hsims.data(), s->hsims.size() * sizeof(SimDev),
+                        hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  if (best_ms) *best_ms = best;
+  if (best_cpp) *best_cpp = bcpp;
+  return check_device_flags(s);
 }
 
 extern "C" int sph_synchronize(sph_sim* s) {
@@ -424,7 +556,7 @@ extern "C" int sph_read_field(sph_sim* s, int32_t field, double* out) {
   if (!s || !out) return fail(SPH_EINVAL, "null argument");
   if (field < 0 || field >= SPH_NFIELDS) return fail(SPH_EINVAL, "unknown field %d", field);
   s->bind();
-  int rc = enqueue_read_field(s->pb, field, s->d_scratch, s->st);
+  int rc = enqueue_read_field(s->pb, field, -1, s->d_scratch, s->st);
   if (rc) return fail(SPH_EDEVICE, "readback launch failed");
   HIPCHK(hipMemcpyAsync(out, s->d_scratch, s->np * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -440,7 +572,7 @@ extern "C" int sph_write_field(sph_sim* s, int32_t field, const double* in) {
       if (!std::isfinite(in[i])) return fail(SPH_ERANGE, "non-finite position");
   s->bind();
   HIPCHK(hipMemcpyAsync(s->d_scratch, in, s->np * sizeof(double), hipMemcpyHostToDevice, s->st));
-  int rc = enqueue_write_field(s->pb, field, s->d_scratch, s->st);
+  int rc = enqueue_write_field(s->pb, field, -1, s->d_scratch, s->st);
   if (rc) return fail(SPH_EDEVICE, "upload launch failed");
   HIPCHK(hipStreamSynchronize(s->st));
   return 0;
@@ -554,6 +686,61 @@ extern "C" int sph_knn_counters(sph_sim* s, int64_t out[7]) {
   const unsigned long long v[7] = {st.retries, st.ties, st.failures, st.miss_lo,
                                    st.miss_hi, st.miss_ann, st.split_fallback};
   for (int i = 0; i < 7; ++i) out[i] = (int64_t)v[i];
+  return 0;
+}
+
+extern "C" int sph_batch_create(int32_t count, const sph_params* params, const int64_t* n,
+                                const double* const* x, const double* const* y,
+                                const double* const* vx, const double* const* vy,
+                                const double* const* e, sph_sim** out) {
+  if (!params || !n || !x || !y || count < 1) return fail(SPH_EINVAL, "bad batch arguments");
+  std::vector<const sph_params*> ps(count);
+  for (int i = 0; i < count; ++i) ps[i] = params + i;
+  return create_batch(count, ps.data(), n, x, y, vx, vy, e, nullptr, out);
+}
+
+extern "C" int32_t sph_batch_count(const sph_sim* s) { return s ? (int32_t)s->hsims.size() : -1; }
+
+extern "C" int64_t sph_batch_num_particles(const sph_sim* s, int32_t sim) {
+  if (!s || sim < 0 || sim >= (int32_t)s->hsims.size()) return -1;
+  return s->sim_n[sim];
+}
+
+extern "C" int sph_batch_read_field(sph_sim* s, int32_t sim, int32_t field, double* out) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  if (sim < 0 || sim >= (int32_t)s->hsims.size()) return fail(SPH_EINVAL, "no simulation %d", sim);
+  if (field < 0 || field >= SPH_NFIELDS) return fail(SPH_EINVAL, "unknown field %d", field);
+  s->bind();
+  if (enqueue_read_field(s->pb, field, sim, s->d_scratch, s->st))
+    return fail(SPH_EDEVICE, "readback launch failed");
+  HIPCHK(hipMemcpyAsync(out, s->d_scratch, s->sim_n[sim] * sizeof(double), hipMemcpyDeviceToHost,
+                        s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int sph_batch_write_field(sph_sim* s, int32_t sim, int32_t field, const double* in) {
+  if (!s || !in) return fail(SPH_EINVAL, "null argument");
+  if (sim < 0 || sim >= (int32_t)s->hsims.size()) return fail(SPH_EINVAL, "no simulation %d", sim);
+  if (field < 0 || field > SPH_VPY || field == SPH_RHO || field == SPH_C)
+    return fail(SPH_EINVAL, "field %d cannot be written", field);
+  s->bind();
+  HIPCHK(hipMemcpyAsync(s->d_scratch, in, s->sim_n[sim] * sizeof(double), hipMemcpyHostToDevice,
+                        s->st));
+  if (enqueue_write_field(s->pb, field, sim, s->d_scratch, s->st))
+    return fail(SPH_EDEVICE, "upload launch failed");
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int sph_batch_reductions(sph_sim* s, int32_t sim, double out[6]) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  if (sim < 0 || sim >= (int32_t)s->hsims.size()) return fail(SPH_EINVAL, "no simulation %d", sim);
+  int rc = fetch_sims(s);
+  if (rc) return rc;
+  const SimDev& d = s->hsims[sim];
+  for (int r = 0; r < kNumRed; ++r) out[r] = d.red[r];
+  out[5] = d.dt_last;
   return 0;
 }
 

@@ -72,8 +72,8 @@ struct PassBuffers {
 int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
                  hipEvent_t* ev /* 2*SPH_NTIMERS or null */);
 
-int enqueue_read_field(const PassBuffers& b, int field, double* dev_out, hipStream_t st);
-int enqueue_write_field(PassBuffers& b, int field, const double* dev_in, hipStream_t st);
+int enqueue_read_field(const PassBuffers& b, int field, int sim, double* dev_out, hipStream_t st);
+int enqueue_write_field(PassBuffers& b, int field, int sim, const double* dev_in, hipStream_t st);
 int enqueue_read_neighbours(const PassBuffers& b, uint32_t* dev_ids, double* dev_d2,
                             hipStream_t st);
 int enqueue_init_sims(PassBuffers& b, hipStream_t st);

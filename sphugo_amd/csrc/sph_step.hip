@@ -1399,31 +1399,38 @@ __device__ inline double* field_ptr(const Positions& p, const Fields& f, int fie
   }
 }
 
-__global__ void k_scatter_by_id(Positions p, Fields f, int field, int64_t np,
-                                double* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  out[f.id[t]] = field_ptr(p, f, field)[t];
+// Field readback / upload by particle ID.  Row of particle t is
+// off(sim) + ID(t); `sim` < 0 selects every simulation, otherwise rows are
+// relative to that simulation's first particle.
+__global__ void k_scatter_by_id(const Item* __restrict__ items, int nitems,
+                                const SimDev* __restrict__ sims, Positions p, Fields f,
+                                int field, int sim, double* __restrict__ out) {
+  const Item it = items[blockIdx.x];
+  if ((int)threadIdx.x >= it.count || (sim >= 0 && it.sim != sim)) return;
+  const int64_t t = it.first + threadIdx.x;
+  const int64_t base = sim >= 0 ? 0 : sims[it.sim].off;
+  out[base + f.id[t]] = field_ptr(p, f, field)[t];
 }
 
-__global__ void k_gather_by_id(Positions p, Fields f, int field, int64_t np,
-                               const double* __restrict__ in) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= np) return;
-  field_ptr(p, f, field)[t] = in[f.id[t]];
+__global__ void k_gather_by_id(const Item* __restrict__ items, int nitems,
+                               const SimDev* __restrict__ sims, Positions p, Fields f,
+                               int field, int sim, const double* __restrict__ in) {
+  const Item it = items[blockIdx.x];
+  if ((int)threadIdx.x >= it.count || (sim >= 0 && it.sim != sim)) return;
+  const int64_t t = it.first + threadIdx.x;
+  const int64_t base = sim >= 0 ? 0 : sims[it.sim].off;
+  field_ptr(p, f, field)[t] = in[base + f.id[t]];
 }
 
-int enqueue_read_field(const PassBuffers& b, int field, double* dev_out, hipStream_t st) {
-  const int64_t nb = (b.np + 255) / 256;
-  hipLaunchKernelGGL(k_scatter_by_id, dim3(nb), dim3(256), 0, st, b.pos_cur, b.fcur, field,
-                     b.np, dev_out);
+int enqueue_read_field(const PassBuffers& b, int field, int sim, double* dev_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_scatter_by_id, dim3(b.nitems), dim3(kItem), 0, st, b.items, b.nitems, b.sims,
+                     b.pos_cur, b.fcur, field, sim, dev_out);
   return (int)hipGetLastError();
 }
 
-int enqueue_write_field(PassBuffers& b, int field, const double* dev_in, hipStream_t st) {
-  const int64_t nb = (b.np + 255) / 256;
-  hipLaunchKernelGGL(k_gather_by_id, dim3(nb), dim3(256), 0, st, b.pos_cur, b.fcur, field,
-                     b.np, dev_in);
+int enqueue_write_field(PassBuffers& b, int field, int sim, const double* dev_in, hipStream_t st) {
+  hipLaunchKernelGGL(k_gather_by_id, dim3(b.nitems), dim3(kItem), 0, st, b.items, b.nitems, b.sims,
+                     b.pos_cur, b.fcur, field, sim, dev_in);
   return (int)hipGetLastError();
 }
 
