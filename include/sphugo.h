@@ -193,6 +193,53 @@ int sph_set_graph(sph_sim* s, int32_t on);
  * before the first step (SPH_ESTATE otherwise).  Either output may be NULL. */
 int sph_autotune(sph_sim* s, double* best_ms, double* best_cells_per_particle);
 
+/* ---- serve loop: continuous batching (no reference counterpart) ----
+ * A native worker thread steps every admitted job in one shared device
+ * pool: each tick is one force pass over all resident jobs, which may be at
+ * different steps of their lives; a job that ends frees its particle range
+ * for the next one in the queue (FIFO, first fit).  A job's results equal
+ * those of sph_create + sph_step(nsteps) with the same inputs.  All calls
+ * are thread-safe. */
+typedef struct sph_server sph_server;
+
+#define SPH_JOB_QUEUED 0
+#define SPH_JOB_RUNNING 1
+#define SPH_JOB_DONE 2
+#define SPH_JOB_FAILED 3
+
+/* capacity: particle slots of the pool; max_jobs: jobs resident at once;
+ * kmax: largest k of a job; precision and flags (SPH_FLAG_BITWISE only) are
+ * shared by all jobs.  Uses the device current on the calling thread. */
+int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kmax, int32_t precision,
+                      uint32_t flags, sph_server** out);
+/* Stops the worker; jobs still queued or running are abandoned. */
+void sph_server_destroy(sph_server* s);
+/* Queue a job of n particles for nsteps >= 1 steps (inputs are copied; vx,
+ * vy, e may be NULL).  IDs are 0..n-1 in input order. */
+int sph_server_submit(sph_server* s, const sph_params* p, int64_t n, const double* x,
+                      const double* y, const double* vx, const double* vy, const double* e,
+                      int32_t nsteps, int64_t* job_id);
+/* Wait for a job (timeout_s < 0: no limit; SPH_ESTATE on timeout). */
+int sph_server_wait(sph_server* s, int64_t job_id, double timeout_s);
+/* Wait until no job is queued or running. */
+int sph_server_drain(sph_server* s);
+/* state (SPH_JOB_*); info = queue wait ms, run ms, submit-to-done latency ms
+ * (-1 where not reached), steps completed. */
+int sph_server_job_info(sph_server* s, int64_t job_id, int32_t* state, double info[4]);
+/* One field of a finished job, indexed by ID: SPH_X, SPH_Y, SPH_VX, SPH_VY,
+ * SPH_RHO, SPH_E or SPH_H. */
+int sph_server_read_result(sph_server* s, int64_t job_id, int32_t field, double* out);
+/* sph_reductions of a finished job, then TotalMomentum (|v| of ID n-1's slot). */
+int sph_server_job_reductions(sph_server* s, int64_t job_id, double out[7]);
+/* Forget a finished or failed job and its results. */
+int sph_server_release(sph_server* s, int64_t job_id);
+/* jobs done, jobs failed, particle-steps done, seconds from the first submit
+ * to the last completion, particle-steps/s over that span, p50 and p99
+ * submit-to-done latency (ms), mean pool occupancy, ticks, jobs queued or
+ * running now. */
+#define SPH_SERVER_NSTATS 10
+int sph_server_stats(sph_server* s, double out[SPH_SERVER_NSTATS]);
+
 const char* sph_last_error(void);
 const char* sph_version(void);
 

@@ -89,6 +89,18 @@ SIGNATURES = {
     "sph_batch_reductions": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
     "sph_set_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "sph_autotune": (ctypes.c_int, [_VP, _D, _D]),
+    "sph_server_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_uint32, ctypes.POINTER(_VP)]),
+    "sph_server_destroy": (None, [_VP]),
+    "sph_server_submit": (ctypes.c_int, [_VP, ctypes.POINTER(Params), ctypes.c_int64, _D, _D, _D,
+                                         _D, _D, ctypes.c_int32, _I64]),
+    "sph_server_wait": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_double]),
+    "sph_server_drain": (ctypes.c_int, [_VP]),
+    "sph_server_job_info": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32), _D]),
+    "sph_server_read_result": (ctypes.c_int, [_VP, ctypes.c_int64, ctypes.c_int32, _D]),
+    "sph_server_job_reductions": (ctypes.c_int, [_VP, ctypes.c_int64, _D]),
+    "sph_server_release": (ctypes.c_int, [_VP, ctypes.c_int64]),
+    "sph_server_stats": (ctypes.c_int, [_VP, _D]),
     "sph_last_error": (ctypes.c_char_p, []),
     "sph_version": (ctypes.c_char_p, []),
 }
@@ -307,3 +319,72 @@ def autotune(sim):
     cpp = ctypes.c_double()
     check(lib().sph_autotune(sim._h, ctypes.byref(ms), ctypes.byref(cpp)))
     return ms.value, cpp.value
+
+
+JOB_STATES = ["queued", "running", "done", "failed"]
+SERVER_STATS = ["jobs_done", "jobs_failed", "particle_steps", "wall_s", "particle_steps_per_s",
+                "p50_ms", "p99_ms", "occupancy", "ticks", "in_flight"]
+RESULT_FIELDS = ["x", "y", "vx", "vy", "rho", "e", "h"]
+
+
+class Server:
+    """Continuous-batching serve loop (sph_server_*): jobs of independent
+    simulations share one device pool; a native worker thread steps them."""
+
+    def __init__(self, capacity, max_jobs=256, kmax=64, precision=PREC_FP64, flags=0):
+        h = _VP()
+        check(lib().sph_server_create(capacity, max_jobs, kmax, precision, flags,
+                                      ctypes.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().sph_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def submit(self, params: Params, st, nsteps):
+        arrs = [None if st.get(k) is None else np.ascontiguousarray(st[k], np.float64)
+                for k in ("x", "y", "vx", "vy", "e")]
+        n = len(arrs[0])
+        jid = ctypes.c_int64()
+        check(lib().sph_server_submit(self._h, ctypes.byref(params), n,
+                                      *[dptr(a) if a is not None else None for a in arrs],
+                                      nsteps, ctypes.byref(jid)))
+        self._n[jid.value] = n
+        return jid.value
+
+    def wait(self, jid, timeout=-1.0):
+        check(lib().sph_server_wait(self._h, jid, float(timeout)))
+
+    def drain(self):
+        check(lib().sph_server_drain(self._h))
+
+    def info(self, jid):
+        state = ctypes.c_int32()
+        out = np.zeros(4)
+        check(lib().sph_server_job_info(self._h, jid, ctypes.byref(state), dptr(out)))
+        return {"state": JOB_STATES[state.value], "queue_ms": out[0], "run_ms": out[1],
+                "latency_ms": out[2], "steps": int(out[3])}
+
+    def result(self, jid, name):
+        out = np.empty(self._n[jid])
+        check(lib().sph_server_read_result(self._h, jid, FIELD[name], dptr(out)))
+        return out
+
+    def reductions(self, jid):
+        out = np.empty(7)
+        check(lib().sph_server_job_reductions(self._h, jid, dptr(out)))
+        return out
+
+    def release(self, jid):
+        check(lib().sph_server_release(self._h, jid))
+        self._n.pop(jid, None)
+
+    def stats(self):
+        out = np.zeros(len(SERVER_STATS))
+        check(lib().sph_server_stats(self._h, dptr(out)))
+        return dict(zip(SERVER_STATS, out.tolist()))

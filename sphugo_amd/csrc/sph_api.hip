@@ -12,15 +12,13 @@
 #include <string>
 #include <vector>
 
-#include "sph_host.hpp"
-#include "sphugo.h"
+#include "sph_sim.hpp"
 
 using namespace sph;
 
-namespace {
-
 thread_local std::string g_err;
 
+namespace sph {
 int fail(int code, const char* fmt, ...) {
   char buf[512];
   va_list ap;
@@ -30,75 +28,7 @@ int fail(int code, const char* fmt, ...) {
   g_err = buf;
   return code;
 }
-
-#define HIPCHK(call)                                                                \
-  do {                                                                              \
-    hipError_t e_ = (call);                                                         \
-    if (e_ != hipSuccess)                                                           \
-      return fail(SPH_EDEVICE, "%s: %s", #call, hipGetErrorString(e_));             \
-  } while (0)
-
-}  // namespace
-
-struct sph_sim {
-  std::vector<SimDev> hsims;
-  std::vector<int64_t> sim_n;
-  std::vector<Item> hitems;
-  int64_t np = 0;
-  int kmax = 0;
-  int prec = 0;
-  int bitwise = 0;
-  bool timing = false;
-  hipStream_t st = nullptr;
-  hipEvent_t ev[2 * SPH_NTIMERS] = {};
-  double t_ms[SPH_NTIMERS] = {};
-  int64_t t_cnt[SPH_NTIMERS] = {};
-  int cur = 0;
-  Fields F[2] = {};
-  Positions P[2] = {};
-  PassBuffers pb = {};
-  int64_t step = 0;
-  std::vector<void*> allocs;
-  double* d_scratch = nullptr;     // np doubles for readback
-  uint32_t* d_nids = nullptr;      // neighbour readback (lazy)
-  double* d_nd2 = nullptr;
-  KnnStats last_stats = {};
-  // graph replay of a regular step, one per field-buffer parity
-  bool graph_on = false;
-  hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  double grid_cpp = 4.0;        // cells per particle of the Morton grid
-  int64_t cell_capacity = 0;    // cstart entries allocated
-
-  template <class T>
-  int alloc(T** p, int64_t count) {
-    void* q = nullptr;
-    size_t bytes = (size_t)(count > 0 ? count : 1) * sizeof(T);
-    if (hipMalloc(&q, bytes) != hipSuccess)
-      return fail(SPH_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
-    allocs.push_back(q);
-    *p = (T*)q;
-    return 0;
-  }
-  void drop_graphs() {
-    for (auto& g : gexec)
-      if (g) {
-        hipGraphExecDestroy(g);
-        g = nullptr;
-      }
-  }
-  ~sph_sim() {
-    drop_graphs();
-    for (void* q : allocs) hipFree(q);
-    for (auto& e : ev)
-      if (e) hipEventDestroy(e);
-    if (st) hipStreamDestroy(st);
-  }
-  void bind() {
-    pb.fcur = F[cur];
-    pb.fnew = F[cur ^ 1];
-    pb.pos_cur = P[0];
-  }
-};
+}  // namespace sph
 
 static bool is_open(const double a[2]) { return a[0] == -DBL_MAX; }
 
@@ -122,7 +52,7 @@ extern "C" int sph_default_params(sph_params* p) {
   return 0;
 }
 
-static int check_params(const sph_params* p, int64_t n) {
+int sph::check_params(const sph_params* p, int64_t n) {
   if (!p) return fail(SPH_EINVAL, "null params");
   if (n < 2) return fail(SPH_EINVAL, "need at least 2 particles, got %lld", (long long)n);
   if (n >= (1LL << 28)) return fail(SPH_EINVAL, "too many particles in one batch: %lld", (long long)n);
@@ -149,13 +79,13 @@ static int check_params(const sph_params* p, int64_t n) {
 // particle at the mean density (Morton-ordered cells; coarser levels of the
 // same grid serve the denser regions' neighbours and the sparser regions'
 // search boxes).
-static int grid_log2(int64_t n, double cpp) {
+int sph::grid_log2(int64_t n, double cpp) {
   int lg = 0;
   while (lg < 13 && (double)(1LL << (2 * lg)) < cpp * (double)n) ++lg;
   return lg;
 }
 
-static void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
+void sph::fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
   const int g = 1 << lg;
   std::memset(&d, 0, sizeof d);
   d.n = (int32_t)n;
@@ -196,7 +126,7 @@ static void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
   d.r_init = 1.2 * std::sqrt(p->k * d.lx * d.ly / (M_PI * (double)n));
 }
 
-static void set_grid(SimDev& d, int lg) {
+void sph::set_grid(SimDev& d, int lg) {
   const int g = 1 << lg;
   d.lg = lg;
   d.gx = g;
@@ -223,8 +153,39 @@ static int64_t layout_cells(sph_sim* h, double cpp) {
   return coff;
 }
 
+// Per-particle device state of a handle holding np particle slots in
+// `nitems` work items (fields, positions, cell ranges, neighbour lists,
+// force records, kNN and sort scratch).  cell_capacity must be set.
+int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
+  PassBuffers& pb = h->pb;
+  int rc = 0;
+  for (int q = 0; q < 2; ++q) {
+    Fields& f = h->F[q];
+    double** fl[] = {&f.vx, &f.vy, &f.e, &f.ax, &f.ay, &f.edot, &f.vpx, &f.vpy, &f.ep, &f.h, &f.rho, &f.c};
+    for (double** a : fl)
+      if ((rc = h->alloc(a, np))) return rc;
+    if ((rc = h->alloc(&f.id, np))) return rc;
+  }
+  if ((rc = h->alloc(&h->P[0].x, np)) || (rc = h->alloc(&h->P[0].y, np)) ||
+      (rc = h->alloc(&pb.xyp, np)))
+    return rc;
+  if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, h->cell_capacity)) ||
+      (rc = h->alloc(&pb.nbr, np * pb.kmax)) ||
+      (rc = h->alloc((char**)&pb.rec,
+                     np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) ||
+      (rc = h->alloc(&pb.partials, (int64_t)nitems * kNumRed)) ||
+      (rc = h->alloc(&pb.stats, 1)) || (rc = h->alloc(&pb.err, 1)) ||
+      (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
+      (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, np * 16)) ||
+      (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
+      (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
+      (rc = h->alloc(&pb.sort.counts, sort_counts_size(np))) || (rc = h->alloc(&pb.sort.tot, 256)) ||
+      (rc = h->alloc(&h->d_scratch, np)))
+    return rc;
+  return 0;
+}
+
 static const double kTuneCpp[] = {0.5, 1.0, 2.0, 4.0, 8.0, 16.0};
-static int fetch_sims(sph_sim* s);
 
 // Build a handle holding `count` simulations.
 static int create_batch(int count, const sph_params* const* ps, const int64_t* ns,
@@ -306,28 +267,11 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   if ((rc = h->alloc(&d_items, pb.nitems)) || (rc = h->alloc(&d_sims, count)) ||
       (rc = h->alloc(&d_rf, count)) || (rc = h->alloc(&d_rc, count)))
     return bail(rc);
-  for (int q = 0; q < 2; ++q) {
-    Fields& f = h->F[q];
-    double** fl[] = {&f.vx, &f.vy, &f.e, &f.ax, &f.ay, &f.edot, &f.vpx, &f.vpy, &f.ep, &f.h, &f.rho, &f.c};
-    for (double** a : fl)
-      if ((rc = h->alloc(a, np))) return bail(rc);
-    if ((rc = h->alloc(&f.id, np))) return bail(rc);
-  }
-  if ((rc = h->alloc(&h->P[0].x, np)) || (rc = h->alloc(&h->P[0].y, np)) ||
-      (rc = h->alloc(&pb.xyp, np)))
-    return bail(rc);
-  if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, h->cell_capacity)) ||
-      (rc = h->alloc(&pb.nbr, np * pb.kmax)) ||
-      (rc = h->alloc((char**)&pb.rec, np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) || (rc = h->alloc(&pb.partials, (int64_t)pb.nitems * kNumRed)) ||
-      (rc = h->alloc(&pb.stats, 1)) || (rc = h->alloc(&pb.err, 1)) ||
-      (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
-      (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, np * 16)) ||
-      (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
-      (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
-      (rc = h->alloc(&pb.sort.counts, sort_counts_size(np))) || (rc = h->alloc(&pb.sort.tot, 256)) ||
-      (rc = h->alloc(&h->d_scratch, np)))
-    return bail(rc);
+  if ((rc = alloc_state(h, np, pb.nitems))) return bail(rc);
   pb.items = d_items;
+  pb.items_all = d_items;
+  pb.nitems_all = pb.nitems;
+  pb.hole_cpp = 0;
   pb.sims = d_sims;
   pb.red_first = d_rf;
   pb.red_count = d_rc;
@@ -419,7 +363,7 @@ static int run_pass(sph_sim* s, int perm_mode, int force_mode) {
   return 0;
 }
 
-static int check_device_flags(sph_sim* s) {
+int sph::check_device_flags(sph_sim* s) {
   int err = 0;
   KnnStats st;
   HIPCHK(hipMemcpyAsync(&err, s->pb.err, sizeof(int), hipMemcpyDeviceToHost, s->st));
@@ -593,7 +537,7 @@ extern "C" int sph_read_neighbours(sph_sim* s, uint32_t* ids, double* d2) {
   return 0;
 }
 
-static int fetch_sims(sph_sim* s) {
+int sph::fetch_sims(sph_sim* s) {
   HIPCHK(hipMemcpyAsync(s->hsims.data(), s->pb.sims, s->hsims.size() * sizeof(SimDev),
                         hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));

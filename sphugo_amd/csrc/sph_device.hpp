@@ -33,6 +33,8 @@ struct SimDev {
   int32_t use_cfl;
   int32_t gx, gy;   // cells per axis (gx = gy = 2^lg)
   int32_t lg;       // log2 of cells per axis (Morton order)
+  int32_t pmode;    // pass modes when a launch runs each simulation in its own
+  int32_t fmode;    // mode (continuous batching): PermuteMode / ForceMode
   int32_t pad1;
   double x0, y0;    // periodic box origin (Hor[0], Vert[0])
   double lx, ly;    // box size (Hor[1]-Hor[0], Vert[1]-Vert[0])

@@ -30,12 +30,16 @@ struct Positions {
   double *x, *y;
 };
 
-enum PermuteMode : int { kPermPredict = 0, kPermInit = 1, kPermKeep = 2 };
+// -1 (kPerSim) selects each simulation's own mode (SimDev::pmode / fmode)
+enum PermuteMode : int { kPerSim = -1, kPermPredict = 0, kPermInit = 1, kPermKeep = 2 };
 enum ForceMode : int { kForceKick = 0, kForceNoKick = 1 };
 
 struct PassBuffers {
-  const Item* items;
+  const Item* items;     // work items of the simulations
   int nitems;
+  const Item* items_all; // plus unused pool ranges (sim = -1), for keys/permute
+  int nitems_all;
+  uint32_t hole_cpp;     // key of an unused slot t is t * hole_cpp
   SimDev* sims;
   int nsims;
   int kmax;             // row stride of the neighbour list (multiple of 4)

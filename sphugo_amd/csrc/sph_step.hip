@@ -47,12 +47,19 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
                                                 const double* __restrict__ VX,
                                                 const double* __restrict__ VY, int mode,
                                                 uint32_t* __restrict__ keys, int* err,
-                                                uint32_t* __restrict__ retry_count) {
+                                                uint32_t* __restrict__ retry_count,
+                                                uint32_t hole_cpp) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *retry_count = 0u;
   const Item it = items[xcd_item(nitems)];
   if ((int)threadIdx.x >= it.count) return;
-  const SimDev* S = sims + it.sim;
   const int64_t t = it.first + threadIdx.x;
+  if (it.sim < 0) {
+    // unused pool slot: a key of its own position keeps it in place
+    keys[t] = (uint32_t)(t * hole_cpp);
+    return;
+  }
+  const SimDev* S = sims + it.sim;
+  if (mode < 0) mode = S->pmode;
   double x = X[t], y = Y[t];
   if (mode == kPermPredict) {
     const double dt = S->dt_half;
@@ -73,9 +80,17 @@ __global__ __launch_bounds__(kItem) void k_permute(
     uint32_t* __restrict__ cstart, int64_t np, int64_t ncells) {
   const Item it = items[xcd_item(nitems)];
   if ((int)threadIdx.x >= it.count) return;
-  const SimDev* S = sims + it.sim;
   const int64_t t = it.first + threadIdx.x;
   const uint32_t i = perm[t];
+  // cell ranges: cells (key[t-1], key[t]] start at t
+  const int64_t kt = skeys[t];
+  const int64_t kp = t > 0 ? (int64_t)skeys[t - 1] : -1;
+  for (int64_t q = kp + 1; q <= kt; ++q) cstart[q] = (uint32_t)t;
+  if (t == np - 1)
+    for (int64_t q = kt + 1; q <= ncells; ++q) cstart[q] = (uint32_t)np;
+  if (it.sim < 0) return;  // unused pool slot
+  const SimDev* S = sims + it.sim;
+  if (mode < 0) mode = S->pmode;
   double x = pin.x[i], y = pin.y[i];
   const double vx = fin.vx[i], vy = fin.vy[i], e = fin.e[i];
   double vpx, vpy, ep;
@@ -104,12 +119,6 @@ __global__ __launch_bounds__(kItem) void k_permute(
   CellIdx c = cell_of(S, x, y);
   const int64_t wx = floordiv(c.ux, S->gx), wy = floordiv(c.uy, S->gy);
   wrap[t] = (uint8_t)((wx + 2) | ((wy + 2) << 3));
-  // cell ranges: cells (key[t-1], key[t]] start at t
-  const int64_t kt = skeys[t];
-  const int64_t kp = t > 0 ? (int64_t)skeys[t - 1] : -1;
-  for (int64_t q = kp + 1; q <= kt; ++q) cstart[q] = (uint32_t)t;
-  if (t == np - 1)
-    for (int64_t q = kt + 1; q <= ncells; ++q) cstart[q] = (uint32_t)np;
 }
 
 // ------------------------------------------------------------------------
@@ -1062,6 +1071,7 @@ __global__ __launch_bounds__(256) void k_force(
   const double lx = S->lx, ly = S->ly;
   const int kern = S->kernel;
   const double dt = S->dt_half;
+  if (mode < 0) mode = S->fmode;
   double xi, yi, hi, ci, rhoi, A;
   double sx = 0.0, sy = 0.0, se = 0.0;
   const uint32_t* row = nbr + t * kstride;
@@ -1271,6 +1281,8 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
                                                 int mode) {
   __shared__ double sh[kNumRed][256];
   SimDev* S = sims + blockIdx.x;
+  if (S->n == 0) return;  // unused slot
+  if (mode < 0) mode = S->fmode;
   const int tid = threadIdx.x;
   const int i0 = item_first[blockIdx.x], ni = item_count[blockIdx.x];
   double acc[kNumRed];
@@ -1315,17 +1327,18 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   };
   const dim3 blk(kItem);
   const dim3 grd(b.nitems);
+  const dim3 grd_all(b.nitems_all);
   mark(0, 0);
-  hipLaunchKernelGGL(k_keys, grd, blk, 0, st, b.items, b.nitems, b.sims, b.pos_cur.x,
+  hipLaunchKernelGGL(k_keys, grd_all, blk, 0, st, b.items_all, b.nitems_all, b.sims, b.pos_cur.x,
                      b.pos_cur.y, b.fcur.vx, b.fcur.vy, perm_mode, b.sort.keys, b.err,
-                     b.retry_count);
+                     b.retry_count, b.hole_cpp);
   mark(0, 1);
   mark(1, 0);
   uint32_t *skeys = nullptr, *perm = nullptr;
   sort_pairs(b.sort, b.np, b.key_bits, st, true, &skeys, &perm);
   mark(1, 1);
   mark(2, 0);
-  hipLaunchKernelGGL(k_permute, grd, blk, 0, st, b.items, b.nitems, b.sims, perm, skeys,
+  hipLaunchKernelGGL(k_permute, grd_all, blk, 0, st, b.items_all, b.nitems_all, b.sims, perm, skeys,
                      perm_mode, b.pos_cur, b.fcur, b.xyp, b.fnew, b.wrap, b.cstart, b.np,
                      b.ncells);
   mark(2, 1);
