@@ -11,6 +11,10 @@ FP32-compensated form; the plain FP64 form is timed beside it.  The dominant
 kernel's roofline uses per-launch HIP event timings taken on the library's
 own stream.  The CPU baseline is the oracle's faithful restatement of the
 reference (tree build + 9-image tree kNN), timed on this host.
+
+The serving workload (`serve` in the line) pushes 1000 jobs with Poisson
+arrivals through the continuous-batching server and reports throughput and
+p50/p99 submit-to-done latency.
 """
 from __future__ import annotations
 
@@ -137,6 +141,118 @@ def cpu_baseline(threads_all: int):
     }
 
 
+SERVE_TEMPLATE_N = 1 << 20
+SERVE_TEMPLATES = 16
+
+
+def serve_ics_factory(seed=5):
+    """Initial conditions of served jobs: 16 clustered templates of 2^20
+    particles (the clustered recipe, template seeds seed*1000+t), shuffled
+    once; job i takes a window of n particles of template t_i from a random
+    start and shifts it by a random periodic offset (all drawn from the job's
+    own seed), so every job has its own clustered layout at a few ms of
+    host time."""
+    base = []
+    for t in range(SERVE_TEMPLATES):
+        st = ics.clustered_pcg(SERVE_TEMPLATE_N, seed=seed * 1000 + t)
+        perm = np.random.default_rng(seed * 1000 + t).permutation(SERVE_TEMPLATE_N)
+        base.append({c: st[c][perm] for c in ("x", "y", "vx", "vy")})
+
+    def make(i, n, k):
+        g = np.random.default_rng(seed * 10**6 + i)
+        b = base[int(g.integers(SERVE_TEMPLATES))]
+        idx = (int(g.integers(SERVE_TEMPLATE_N)) + np.arange(n)) % SERVE_TEMPLATE_N
+        sx, sy = g.random(2)
+        x = ics.wrap_unit(b["x"][idx] + sx)
+        y = ics.wrap_unit(b["y"][idx] + sy)
+        return {"x": x, "y": y, "vx": b["vx"][idx], "vy": b["vy"][idx], "e": np.ones(n),
+                "mass": 1.0 / n, "k": k, "n": n}
+
+    return make
+
+
+def run_serve(njobs, precision, capacity, max_jobs=256, load=0.7):
+    """Poisson arrivals of jobs (N log-uniform in [1024, 2^20], k in {32, 64},
+    steps uniform in [10, 200]) into the continuous-batching server at `load`
+    times its capacity measured on a burst of 64 jobs of the same mix."""
+    import queue
+    import threading
+
+    make = serve_ics_factory()
+    prec = L.PREC_FP32C if precision == "fp32c" else L.PREC_FP64
+
+    def submit(srv, i, n, k, steps):
+        st = make(i, n, k)
+        return srv.submit(params_for(st, prec), st, steps)
+
+    # capacity: a burst of 64 jobs, submit to last completion
+    srv = L.Server(capacity, max_jobs=max_jobs, kmax=64, precision=prec)
+    cal = ics.serve_jobs(64, seed=6)
+    ids = [submit(srv, 10**5 + i, n, k, s) for i, (n, k, s) in enumerate(cal)]
+    srv.drain()
+    cs = srv.stats()
+    for j in ids:
+        srv.release(j)
+    srv.close()
+    cap_rate = cs["particle_steps"] / cs["wall_s"]
+
+    jobs = ics.serve_jobs(njobs, seed=5)
+    mean_ps = float(np.mean([n * s for n, _, s in jobs]))
+    lam = load * cap_rate / mean_ps
+    arrivals = np.cumsum(np.random.default_rng(5).exponential(1.0 / lam, njobs))
+    srv = L.Server(capacity, max_jobs=max_jobs, kmax=64, precision=prec)
+    done = queue.Queue()
+    lat = []
+    errs = []
+
+    def reaper():
+        while True:
+            jid = done.get()
+            if jid is None:
+                return
+            try:
+                srv.wait(jid)
+                lat.append(srv.info(jid)["latency_ms"])
+                srv.release(jid)
+            except Exception as exc:  # noqa: BLE001 - reported below
+                errs.append(str(exc))
+
+    th = threading.Thread(target=reaper)
+    th.start()
+    t0 = time.perf_counter()
+    for i, (n, k, s) in enumerate(jobs):
+        st = make(i, n, k)
+        p = params_for(st, prec)
+        wait = t0 + arrivals[i] - time.perf_counter()
+        if wait > 0:
+            time.sleep(wait)
+        done.put(srv.submit(p, st, s))
+        if i % 100 == 99:
+            print(f"# serve: {i + 1} jobs submitted at {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+    srv.drain()
+    done.put(None)
+    th.join()
+    rs = srv.stats()
+    srv.close()
+    if errs:
+        raise RuntimeError(f"serve: {len(errs)} jobs failed: {errs[:3]}")
+    return {
+        "jobs": njobs, "precision": precision, "pool_capacity": capacity, "max_jobs": max_jobs,
+        "capacity_particle_steps_per_s": round(cap_rate, 1),
+        "offered_load": load, "lambda_jobs_per_s": round(lam, 3),
+        "mean_particle_steps_per_job": round(mean_ps, 1),
+        "throughput_particle_steps_per_s": round(rs["particle_steps_per_s"], 1),
+        "p50_ms": round(rs["p50_ms"], 2), "p99_ms": round(rs["p99_ms"], 2),
+        "mean_latency_ms": round(float(np.mean(lat)), 2),
+        "mean_occupancy": round(rs["occupancy"], 4), "ticks": int(rs["ticks"]),
+        "wall_s": round(rs["wall_s"], 2),
+        "ics": "clustered recipe (16 Gaussians sigma 0.02 holding 80%, 20% uniform, "
+               "v ~ N(0, 0.1^2), e = 1): per-job window of one of 16 shuffled 2^20 templates "
+               "with a per-job periodic shift",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +265,10 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay steps from a hipGraph")
     ap.add_argument("--tune", action="store_true", help="autotune the grid before stepping")
     ap.add_argument("--batch-count", type=int, default=0)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--jobs", type=int, default=1000, help="served jobs")
+    ap.add_argument("--serve-precision", default="fp32c", choices=["fp32c", "fp64"])
+    ap.add_argument("--serve-capacity", type=int, default=1 << 23, help="pool particle slots")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -207,6 +327,9 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         line["cpu_baseline"] = cpu_baseline(threads)
+    if not args.no_serve and args.workload == "clustered":
+        line["serve"] = run_serve(args.jobs, args.serve_precision, args.serve_capacity)
+        print(f"# serve: {line['serve']}", file=sys.stderr, flush=True)
     print(json.dumps(line), flush=True)
 
 

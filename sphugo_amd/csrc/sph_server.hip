@@ -114,6 +114,7 @@ struct sph_server {
   // statistics
   int64_t n_done = 0, n_failed = 0, particle_steps = 0, ticks = 0;
   double occupancy_sum = 0;
+  double tick_ms_sum = 0;
   bool have_t0 = false;
   Clock::time_point t0, t_last;
   std::vector<double> lat_ms;
@@ -179,11 +180,21 @@ int upload_items(sph_server* s) {
     push_items(s->h_items, q, j->off, j->n);
     s->h_rc[q] = (int)s->h_items.size() - s->h_rf[q];
   }
+  // keys, sort and permute cover [0, hi), hi = end of the highest job
+  // (first fit keeps jobs low in the pool)
+  int64_t hi = 0;
+  for (Job* j : s->slots)
+    if (j && j->off + j->n > hi) hi = j->off + j->n;
   s->h_items_all = s->h_items;
-  for (auto& h : s->holes) push_items(s->h_items_all, -1, h.first, h.second);
+  for (auto& h : s->holes)
+    if (h.first < hi) push_items(s->h_items_all, -1, h.first, std::min(h.second, hi - h.first));
   PassBuffers& pb = s->core.pb;
   pb.nitems = (int)s->h_items.size();
   pb.nitems_all = (int)s->h_items_all.size();
+  pb.np = hi;
+  int bits = 1;
+  while ((1LL << bits) <= hi * (int64_t)kHoleCpp) ++bits;
+  pb.key_bits = bits;
   const hipStream_t st = s->core.st;
   HIPCHK(hipMemcpyAsync(s->d_items, s->h_items.data(), s->h_items.size() * sizeof(Item),
                         hipMemcpyHostToDevice, st));
@@ -293,6 +304,7 @@ bool tick(sph_server* s) {
       admitted.push_back(j);
     }
   }
+  const Clock::time_point t_tick = Clock::now();
   int rc = 0;
   for (int q : s->to_clear)
     if (!rc && !s->slots[q]) rc = clear_slot(s, q);
@@ -341,6 +353,7 @@ bool tick(sph_server* s) {
   }
   s->ticks += 1;
   s->occupancy_sum += (double)occupied / (double)s->cap;
+  s->tick_ms_sum += ms_between(t_tick, Clock::now());
   const bool bad = s->h_err != 0 || s->h_stats.failures != s->seen_failures;
   if (bad) {
     // the flags are pool-wide: every job of this pass is failed
@@ -633,5 +646,6 @@ extern "C" int sph_server_stats(sph_server* s, double out[SPH_SERVER_NSTATS]) {
   out[7] = s->ticks ? s->occupancy_sum / s->ticks : 0.0;
   out[8] = (double)s->ticks;
   out[9] = (double)(queued + running);
+  out[10] = s->ticks ? s->tick_ms_sum / s->ticks : 0.0;
   return 0;
 }

@@ -77,19 +77,25 @@ __global__ __launch_bounds__(kItem) void k_permute(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys, int mode,
     Positions pin, Fields fin, double2* __restrict__ xyp, Fields fout, uint8_t* __restrict__ wrap,
-    uint32_t* __restrict__ cstart, int64_t np, int64_t ncells) {
+    uint32_t* __restrict__ cstart) {
   const Item it = items[xcd_item(nitems)];
   if ((int)threadIdx.x >= it.count) return;
+  if (it.sim < 0) return;  // unused pool slot: stays in place, owns no cells
   const int64_t t = it.first + threadIdx.x;
   const uint32_t i = perm[t];
-  // cell ranges: cells (key[t-1], key[t]] start at t
-  const int64_t kt = skeys[t];
-  const int64_t kp = t > 0 ? (int64_t)skeys[t - 1] : -1;
-  for (int64_t q = kp + 1; q <= kt; ++q) cstart[q] = (uint32_t)t;
-  if (t == np - 1)
-    for (int64_t q = kt + 1; q <= ncells; ++q) cstart[q] = (uint32_t)np;
-  if (it.sim < 0) return;  // unused pool slot
   const SimDev* S = sims + it.sim;
+  // cell ranges of this simulation's cells [coff, coff + G^2]: cells
+  // (key[t-1], key[t]] start at t; the last particle closes the range.
+  // Cells outside every simulation's range are never read.
+  {
+    const int64_t kt = skeys[t];
+    const int64_t lo = t == S->off ? S->coff : (int64_t)skeys[t - 1] + 1;
+    for (int64_t q = lo; q <= kt; ++q) cstart[q] = (uint32_t)t;
+    if (t == S->off + S->n - 1) {
+      const int64_t end = S->coff + (int64_t)S->gx * S->gy;
+      for (int64_t q = kt + 1; q <= end; ++q) cstart[q] = (uint32_t)(t + 1);
+    }
+  }
   if (mode < 0) mode = S->pmode;
   double x = pin.x[i], y = pin.y[i];
   const double vx = fin.vx[i], vy = fin.vy[i], e = fin.e[i];
@@ -1339,8 +1345,7 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   mark(1, 1);
   mark(2, 0);
   hipLaunchKernelGGL(k_permute, grd_all, blk, 0, st, b.items_all, b.nitems_all, b.sims, perm, skeys,
-                     perm_mode, b.pos_cur, b.fcur, b.xyp, b.fnew, b.wrap, b.cstart, b.np,
-                     b.ncells);
+                     perm_mode, b.pos_cur, b.fcur, b.xyp, b.fnew, b.wrap, b.cstart);
   mark(2, 1);
   mark(3, 0);
   {

@@ -144,3 +144,36 @@ def batch_sizes(count: int = 512, seed: int = 4, lo: int = 1024, hi: int = 65536
 def batch(count: int = 512, seed: int = 4, k: int = 32):
     sizes = batch_sizes(count, seed)
     return [blast(int(n), seed=seed * 10**6 + i, k=k) for i, n in enumerate(sizes)]
+
+
+def clustered_pcg(n: int, seed: int, k: int = 64, nclusters: int = 16, sigma: float = 0.02,
+                  frac: float = 0.8, vsigma: float = 0.1):
+    """The clustered recipe drawn with numpy's PCG64 generator (several times
+    faster than the SplitMix stream above): used where many independent
+    initial conditions are made on the fly, e.g. one per served job."""
+    g = np.random.default_rng(seed)
+    cx = g.random(nclusters)
+    cy = g.random(nclusters)
+    nc = int(frac * n)
+    which = np.arange(nc) % nclusters
+    x = np.empty(n)
+    y = np.empty(n)
+    x[:nc] = cx[which] + sigma * g.standard_normal(nc)
+    y[:nc] = cy[which] + sigma * g.standard_normal(nc)
+    x[nc:] = g.random(n - nc)
+    y[nc:] = g.random(n - nc)
+    x = wrap_unit(x)
+    y = wrap_unit(y)
+    vx = vsigma * g.standard_normal(n)
+    vy = vsigma * g.standard_normal(n)
+    return _state(x, y, vx=vx, vy=vy, k=k, seed=seed, name=f"clustered-pcg-{n}")
+
+
+def serve_jobs(count: int = 1000, seed: int = 5, lo: int = 1024, hi: int = 1 << 20):
+    """Job mix of the serving workload: N log-uniform in [lo, hi], k in
+    {32, 64}, steps uniform in [10, 200]."""
+    g = np.random.default_rng(seed)
+    n = np.floor(np.exp(g.uniform(math.log(lo), math.log(hi), count))).astype(np.int64)
+    k = g.choice(np.array([32, 64]), count)
+    steps = g.integers(10, 201, count)
+    return [(int(a), int(b), int(c)) for a, b, c in zip(n, k, steps)]
