@@ -78,7 +78,11 @@ extern "C" {
 #define SPH_T_KNN_DENSITY 3
 #define SPH_T_FORCE 4
 #define SPH_T_REDUCE 5
-#define SPH_NTIMERS 6
+/* the three launches inside SPH_T_KNN_DENSITY */
+#define SPH_T_KNN_WALK 6     /* tile walk of the bracketed fast path */
+#define SPH_T_KNN_FINISH 7   /* annulus selection, density, force records */
+#define SPH_T_KNN_FALLBACK 8 /* exact wavefront search of the remaining queries */
+#define SPH_NTIMERS 9
 
 /* Values the step reads from SphConfig (sim/config-parser.go:111-149) plus
  * the parameters the reference hard-codes (sim/core.go:14 NN_SIZE,

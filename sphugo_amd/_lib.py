@@ -25,7 +25,8 @@ FLAG_BITWISE, FLAG_TIMING = 1, 2
 FIELDS = ["x", "y", "vx", "vy", "rho", "c", "e", "edot", "ax", "ay", "epred",
           "vpx", "vpy", "h"]
 FIELD = {n: i for i, n in enumerate(FIELDS)}
-TIMERS = ["keys", "sort", "permute", "knn_density", "force", "reduce"]
+TIMERS = ["keys", "sort", "permute", "knn_density", "force", "reduce", "knn_walk",
+          "knn_finish", "knn_fallback"]
 ERRORS = {-1: "EINVAL", -2: "EDEVICE", -3: "ENOMEM", -4: "ESTATE", -5: "ERANGE"}
 
 

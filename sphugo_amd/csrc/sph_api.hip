@@ -170,7 +170,7 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
       (rc = h->alloc(&pb.xyp, np)))
     return rc;
   if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, h->cell_capacity)) ||
-      (rc = h->alloc(&pb.nbr, np * pb.kmax)) ||
+      (rc = h->alloc(&pb.nbr, nbr_entries(np, pb.kmax))) ||
       (rc = h->alloc((char**)&pb.rec,
                      np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) ||
       (rc = h->alloc(&pb.partials, (int64_t)nitems * kNumRed)) ||
@@ -293,7 +293,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   }
   if (hipMemsetAsync(pb.stats, 0, sizeof(KnnStats), h->st) != hipSuccess ||
       hipMemsetAsync(pb.err, 0, sizeof(int), h->st) != hipSuccess ||
-      hipMemsetAsync(pb.nbr, 0, np * pb.kmax * sizeof(uint32_t), h->st) != hipSuccess)
+      hipMemsetAsync(pb.nbr, 0, nbr_entries(np, pb.kmax) * sizeof(uint32_t), h->st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "memset failed"));
   std::vector<uint32_t> idbuf;
   for (int s = 0; s < count; ++s) {

@@ -73,6 +73,9 @@ struct PassBuffers {
 // Enqueue one force pass (keys, sort, permute, kNN + density, force,
 // reductions) on `st`.  Timing events (may be null) are recorded around
 // each kernel class.
+// u32 entries of the neighbour lists of np queries (64-query tiles)
+inline int64_t nbr_entries(int64_t np, int kstride) { return (np + 63) / 64 * 64 * kstride; }
+
 int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
                  hipEvent_t* ev /* 2*SPH_NTIMERS or null */);
 

@@ -469,7 +469,7 @@ extern "C" int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kma
   if (hipMemsetAsync(d_sims, 0, max_jobs * sizeof(SimDev), c.st) != hipSuccess ||
       hipMemsetAsync(pb.stats, 0, sizeof(KnnStats), c.st) != hipSuccess ||
       hipMemsetAsync(pb.err, 0, sizeof(int), c.st) != hipSuccess ||
-      hipMemsetAsync(pb.nbr, 0, capacity * pb.kmax * sizeof(uint32_t), c.st) != hipSuccess ||
+      hipMemsetAsync(pb.nbr, 0, nbr_entries(capacity, pb.kmax) * sizeof(uint32_t), c.st) != hipSuccess ||
       hipStreamSynchronize(c.st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "server init failed"));
   s->slots.assign(max_jobs, nullptr);

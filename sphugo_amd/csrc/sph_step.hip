@@ -208,6 +208,22 @@ __device__ inline bool cand_image(int ptx, int pty, uint8_t wc, int& ix, int& iy
   return nx >= -1 && nx <= 1 && ny >= -1 && ny <= 1;
 }
 
+// Neighbour lists are stored in tiles of 64 consecutive queries: entry e of
+// query t lives at tile(t) * kstride * 64 + (e / 4) * 256 + (t % 64) * 4 +
+// e % 4 (u32 units), so 64 lanes reading the same 4-entry chunk of their
+// rows touch one contiguous 1 KiB segment.  kstride is a multiple of 4.
+struct NbrRow {
+  uint32_t* base;
+  __device__ uint32_t& operator[](int e) const { return base[(e >> 2) * 256 + (e & 3)]; }
+  __device__ u32x4* chunk(int c) const { return (u32x4*)(base + c * 256); }
+};
+__device__ inline NbrRow nbr_row(uint32_t* nbr, int64_t t, int kstride) {
+  return NbrRow{nbr + (t >> 6) * (int64_t)kstride * 64 + (t & 63) * 4};
+}
+__device__ inline NbrRow nbr_row(const uint32_t* nbr, int64_t t, int kstride) {
+  return nbr_row(const_cast<uint32_t*>(nbr), t, kstride);
+}
+
 // Exact kNN of one query by the whole wavefront, starting from radius R.
 // Writes the k entries of `row` (BITWISE: sorted descending by (d2, ID,
 // image), the reference's queue order) and returns tau and the density
@@ -217,7 +233,7 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
                                  const double2* __restrict__ XY,
                                  const uint8_t* __restrict__ WR,
                                  const uint32_t* __restrict__ cstart,
-                                 const uint32_t* __restrict__ ID, uint32_t* __restrict__ row,
+                                 const uint32_t* __restrict__ ID, NbrRow row,
                                  SearchShared& sh, unsigned long long& nretry,
                                  unsigned long long& nties, unsigned long long& nfail) {
   const int lane = lane_id();
@@ -596,7 +612,7 @@ __global__ __launch_bounds__(256) void k_knn(
   const double2 pq = XY[t];
   const double xq = pq.x, yq = pq.y;
   const double hp = H[t];
-  uint32_t* row = nbr + t * kstride;
+  const NbrRow row = nbr_row(nbr, t, kstride);
   bool fast = !BITWISE && valid && hp > 0;
   bool resolved = false;
   double tau = 0.0;
@@ -607,7 +623,6 @@ __global__ __launch_bounds__(256) void k_knn(
     int c_lo = 0, c_an = 0;
     // certain neighbours collect in cbuf and leave in 16-byte stores
     u32x4 cbuf = {0u, 0u, 0u, 0u};
-    u32x4* row4w = (u32x4*)row;
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const uint32_t tu = (uint32_t)t;
     // Lane groups: aligned blocks of consecutive queries.  A block whose
@@ -774,7 +789,7 @@ __global__ __launch_bounds__(256) void k_knn(
       cbuf.y = sl_ == 1 ? pkv_ : cbuf.y;                                                \
       cbuf.z = sl_ == 2 ? pkv_ : cbuf.z;                                                \
       cbuf.w = sl_ == 3 ? pkv_ : cbuf.w;                                                \
-      if (sl_ == 3 && c_lo < k) row4w[c_lo >> 2] = cbuf;                                \
+      if (sl_ == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;                                  \
     }                                                                                   \
     if (isa_ && c_an < kAnn) ts.ann_pk[c_an][lane] = pkv_;                              \
     c_lo += isc_ ? 1 : 0;                                                               \
@@ -825,7 +840,7 @@ __global__ __launch_bounds__(256) void k_knn(
     }
     if (fast && c_lo < k && c_an <= kAnn && c_lo + c_an >= k) {
       if (c_lo & 3) {
-        uint32_t* r0 = row + (c_lo & ~3);
+        uint32_t* r0 = &row[c_lo & ~3];
         r0[0] = cbuf.x;
         if ((c_lo & 3) > 1) r0[1] = cbuf.y;
         if ((c_lo & 3) > 2) r0[2] = cbuf.z;
@@ -896,7 +911,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     const double2 pq = XY[t];
     const double xq = pq.x, yq = pq.y;
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
-    uint32_t* row = nbr + t * kstride;
+    const NbrRow row = nbr_row(nbr, t, kstride);
     // annulus squared distances, recomputed exactly as in the walk
     const uint32_t* ai = ann + t * kAnn;
     uint32_t ap[kAnn];
@@ -968,14 +983,13 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     // density: the row holds the k neighbours; fetch them 8 at a time
     const double h = sqrt(tau);
     const int kern = S->kernel;
-    const u32x4* row4 = (const u32x4*)row;
     double acc = 0.0;
     float s_f = 0.f, c_f = 0.f;
     const float hinv = 1.0f / (float)h;
     const double hinv64 = 1.0 / h;
     for (int n0 = 0; n0 < k; n0 += 8) {
-      const u32x4 qa = row4[n0 >> 2];
-      const u32x4 qb = n0 + 4 < k ? row4[(n0 >> 2) + 1] : qa;
+      const u32x4 qa = *row.chunk(n0 >> 2);
+      const u32x4 qb = n0 + 4 < k ? *row.chunk((n0 >> 2) + 1) : qa;
       const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
       double2 pj[8];
 #pragma unroll
@@ -1044,7 +1058,7 @@ __global__ __launch_bounds__(256) void k_knn_fallback(
     const double xq = pq.x, yq = pq.y, hp = H[t];
     const double R0 = hp > 0 ? hp * 1.1 : local_radius(S, cstart, xq, yq);
     const SearchOut so = wave_search<BITWISE, FP32C>(S, t, xq, yq, R0, XY, WR, cstart, ID,
-                                                     nbr + t * kstride, sh, nretry, nties, nfail);
+                                                     nbr_row(nbr, t, kstride), sh, nretry, nties, nfail);
     if (lane == 0 && so.tau > 0)
       write_result<FP32C>(S, t, xq, yq, so.tau, so.facc, EP, VPX, VPY, H, RHO, C, recp);
   }
@@ -1080,8 +1094,7 @@ __global__ __launch_bounds__(256) void k_force(
   if (mode < 0) mode = S->fmode;
   double xi, yi, hi, ci, rhoi, A;
   double sx = 0.0, sy = 0.0, se = 0.0;
-  const uint32_t* row = nbr + t * kstride;
-  const u32x4* row4 = (const u32x4*)row;
+  const NbrRow row = nbr_row(nbr, t, kstride);
   if constexpr (!FP32C) {
     const RecD* rec = (const RecD*)recp;
     const RecD me = rec[t];
@@ -1090,7 +1103,7 @@ __global__ __launch_bounds__(256) void k_force(
     const double alpha = S->alpha, beta = S->beta, eta2 = S->eta2;
     const double hinv = EXACT ? 0.0 : 1.0 / hi;
     for (int n4 = 0; n4 < k; n4 += 4) {
-      const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+      const u32x4 q4 = __builtin_nontemporal_load(row.chunk(n4 >> 2));
       const uint32_t pks[4] = {q4.x, q4.y, q4.z, q4.w};
       RecD nbs[4];
 #pragma unroll
@@ -1154,7 +1167,7 @@ __global__ __launch_bounds__(256) void k_force(
     // Kahan-compensated sums
     float fx = 0.f, fy = 0.f, fe = 0.f, cx = 0.f, cy = 0.f, ce = 0.f;
     for (int n4 = 0; n4 < k; n4 += 4) {
-      const u32x4 q4 = __builtin_nontemporal_load(row4 + (n4 >> 2));
+      const u32x4 q4 = __builtin_nontemporal_load(row.chunk(n4 >> 2));
       const uint32_t pks[4] = {q4.x, q4.y, q4.z, q4.w};
       RecF nbs[4];
 #pragma unroll
@@ -1361,8 +1374,13 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
                      b.nitems, b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h,                       \
                      b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr, b.kmax, b.fnew.rho,   \
                      b.fnew.c, b.rec, b.stats, b.retry, b.retry_count, fastpath ? 0 : 1)
+    mark(6, 0);
     if (fastpath) {
       if (b.prec == kPrecFP32C) SPH_KNN(true); else SPH_KNN(false);
+    }
+    mark(6, 1);
+    mark(7, 0);
+    if (fastpath) {
       if (b.prec == kPrecFP32C)
         hipLaunchKernelGGL((k_knn_finish<true>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
                            b.xyp, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
@@ -1372,9 +1390,12 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
                            b.xyp, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
                            b.kmax, b.fnew.rho, b.fnew.c, b.rec, b.knn_cnt, b.knn_ann, b.stats);
     }
+    mark(7, 1);
+    mark(8, 0);
     if (b.bitwise) SPH_FB(true, false);
     else if (b.prec == kPrecFP32C) SPH_FB(false, true);
     else SPH_FB(false, false);
+    mark(8, 1);
 #undef SPH_KNN
 #undef SPH_FB
   }
@@ -1468,7 +1489,7 @@ __global__ void k_read_nbrs(const Item* __restrict__ items, int nitems,
   double* rd = d2o + row * kout;
   const double xi = XY[t].x, yi = XY[t].y;
   for (int n = 0; n < k; ++n) {
-    const uint32_t pk = nbr[t * kstride + n];
+    const uint32_t pk = nbr_row(nbr, t, kstride)[n];
     const uint32_t j = pk >> 4;
     const double ox = (double)img_ix(pk & 15u) * S->lx;
     const double oy = (double)img_iy(pk & 15u) * S->ly;
