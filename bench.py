@@ -253,6 +253,57 @@ def run_serve(njobs, precision, capacity, max_jobs=256, load=0.7):
     }
 
 
+def error_report():
+    """FP32C against plain FP64 on the same inputs: density and acceleration
+    after the first force pass (max and rms relative error; acceleration
+    relative to max |a|), neighbour sets, and total-energy drift
+    E = sum m (e + v^2/2) over a run.  The plain FP64 form is the oracle's
+    within 1e-12 (tests/test_gpu_parity.py), so it stands in for the oracle
+    at sizes the CPU restatement cannot run quickly."""
+    cases = [("uniform-2200", ics.uniform(2200, seed=12345678, k=32), 10),
+             ("blast-262144", ics.blast(262144, seed=2, k=32), 100),
+             ("clustered-4194304", ics.clustered(4194304, seed=3, k=64), 50)]
+    out = {}
+    for name, st, steps in cases:
+        res = {}
+        sims = {}
+        for prec in ("fp64", "fp32c"):
+            p = params_for(st, L.PREC_FP32C if prec == "fp32c" else L.PREC_FP64)
+            s = L.Sim(p, st["x"], st["y"], st["vx"], st["vy"], st["e"])
+            s.set("vpx", st["vx"])
+            s.set("vpy", st["vy"])
+            s.set("epred", st["e"])
+            s.calculate_forces()
+            sims[prec] = {f: s.get(f) for f in ("rho", "ax", "ay", "h")}
+            sims[prec]["ids"] = s.neighbours(with_d2=False)[0]
+            s.close()
+            s = L.Sim(p, st["x"], st["y"], st["vx"], st["vy"], st["e"])
+            s.step(1)
+            e0 = s.reductions()[2]
+            s.step(steps - 1)
+            e1 = s.reductions()[2]
+            sims[prec]["drift"] = (e1 - e0) / abs(e0)
+            sims[prec]["E_end"] = e1
+            s.close()
+        a, b = sims["fp32c"], sims["fp64"]
+        rr = np.abs(a["rho"] - b["rho"]) / b["rho"]
+        amag = np.sqrt(b["ax"] ** 2 + b["ay"] ** 2)
+        ar = np.sqrt((a["ax"] - b["ax"]) ** 2 + (a["ay"] - b["ay"]) ** 2) / amag.max()
+        res["neighbour_sets_equal"] = bool((a["ids"] == b["ids"]).all())
+        res["h_equal"] = bool((a["h"] == b["h"]).all())
+        res["rho_rel_max"] = float(rr.max())
+        res["rho_rel_rms"] = float(np.sqrt(np.mean(rr ** 2)))
+        res["accel_rel_max"] = float(ar.max())
+        res["accel_rel_rms"] = float(np.sqrt(np.mean(ar ** 2)))
+        res["steps"] = steps
+        res["energy_drift_fp64"] = float(b["drift"])
+        res["energy_drift_fp32c"] = float(a["drift"])
+        res["energy_end_rel_diff"] = float(abs(a["E_end"] - b["E_end"]) / abs(b["E_end"]))
+        out[name] = res
+        print(f"# errors {name}: {res}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,6 +317,8 @@ def main():
     ap.add_argument("--tune", action="store_true", help="autotune the grid before stepping")
     ap.add_argument("--batch-count", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--errors", action="store_true",
+                    help="print the FP32C error report (FP32C vs FP64) and exit")
     ap.add_argument("--jobs", type=int, default=1000, help="served jobs")
     ap.add_argument("--serve-precision", default="fp32c", choices=["fp32c", "fp64"])
     ap.add_argument("--serve-capacity", type=int, default=1 << 23, help="pool particle slots")
@@ -274,6 +327,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design, not a run)"}))
         sys.exit(2)
+    if args.errors:
+        print(json.dumps({"fp32c_error_report": error_report()}), flush=True)
+        return
     st = make_state(args)
     k = st["k"]
     res = {}
