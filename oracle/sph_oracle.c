@@ -9,7 +9,10 @@
  * own and evaluates left to right, and so does this file.
  *
  * Parity status of each part (how it is pinned) is recorded in the test
- * modules tests/test_oracle_*.py.
+ * modules tests/test_oracle_*.py and tests/test_golden.py.  One function is
+ * parity unpinned: orc_hypot (Go's math.Hypot rounding, which only Go could
+ * pin); it only steers the faithful tree's pruning, never the exact kNN the
+ * device path is checked against.
  */
 #include "sph_oracle.h"
 

@@ -1,17 +1,29 @@
 // SPH force pass and integrator kernels for gfx950 (wave64).
 //
-//   k_keys     row-major cell key of each particle's pass position
-//              (replaces Treebuild/Partition, sim/core.go:126-224)
-//   k_permute  gather the state into cell order, apply the drift-1 /
-//              predictor step (sim/sph.go:108-117) or the step-0
-//              initialisation (:96-101), build the cell ranges
-//   k_knn      exact kNN, one wavefront per query, fused with density and
-//              sound speed (sim/nearest-neighbour.go:28-165,
-//              sim/sph.go:306-323,421-429)
-//   k_force    pressure + artificial viscosity + EDot, one lane group per
-//              particle, fused with kick, drift 2, periodic wrap and
-//              reflection (sim/sph.go:327-401,122-193)
-//   k_reduce   per-simulation FP64 totals and the CFL half step
+//   k_keys          cell key of each particle's pass position: Morton code
+//                   of a power-of-two grid plus the simulation's cell offset
+//                   (replaces Treebuild/Partition, sim/core.go:126-224)
+//   sort_pairs      stable LSD radix sort of (key, index) (sph_sort.hip)
+//   k_permute       gather the state into cell order, apply the drift-1 /
+//                   predictor step (sim/sph.go:108-117) or the step-0
+//                   initialisation (:96-101), write the cell ranges
+//   k_knn           tile walk: a wavefront takes 64 consecutive queries,
+//                   stages the candidates of their cells through LDS and
+//                   sorts them against each query's bracket around its last
+//                   h: certain neighbours go to the list, the annulus to a
+//                   short side list; misses go to a retry list
+//   k_knn_finish    thread per query: exact selection of the annulus,
+//                   density and sound speed, the force record
+//                   (sim/nearest-neighbour.go:28-165, sim/sph.go:306-323,
+//                   421-429)
+//   k_knn_fallback  exact wavefront-per-query search for the retry list (or
+//                   for every query: first pass, bitwise order)
+//   k_force         thread per particle: pressure + artificial viscosity +
+//                   EDot over the packed records of its neighbours, fused
+//                   with kick, drift 2, periodic wrap and reflection
+//                   (sim/sph.go:327-401,122-193), partial sums and the CFL
+//                   minimum
+//   k_reduce        per-simulation FP64 totals and the next CFL half step
 //
 // All FP64 arithmetic on the parity path is evaluated in the reference's
 // association order; the library is compiled with -ffp-contract=off so no
