@@ -37,16 +37,32 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "particle-steps/s at 4M (FP32-comp vs plain FP64); p50/p99 job latency in serve"
 
 
-def algo_bytes(kernel: str, k: int) -> int:
-    """Algorithmic HBM bytes per particle of one launch (state arrays 8 B,
-    IDs and cell keys 4 B, neighbour entries 4 B; neighbour gathers count
-    as on-chip reuse)."""
+def algo_bytes(kernel: str, k: int, precision: str = "fp32c") -> int:
+    """Algorithmic HBM bytes per particle of one launch: own-particle streams
+    read or written once (state 8 B, IDs and keys 4 B), materialised
+    neighbour lists 4 B per entry; neighbour gathers count as on-chip reuse."""
+    rec = 48 if precision == "fp32c" else 64
     return {
-        "keys": 32 + 4,                    # x, y, vx, vy -> key
-        "permute": 4 + 4 + 9 * 8 + 4 + 10 * 8 + 4 + 1,  # perm, key, gather, write
-        "knn_density": 16 + 1 + 8 + 8 + 4 * k + 3 * 8,  # pos, wrap, h_prev, e_pred -> list, h, rho, c
-        "force": 4 * k + 10 * 8 + 8 * 8,   # list, own state -> a, edot, v, e, x, y
+        "keys": 32 + 4,                                   # x, y, vx, vy -> key
+        "permute": 4 + 4 + 9 * 8 + 4 + 10 * 8 + 4 + 1,    # perm, key, gather, write
+        "knn_walk": 16 + 8 + 4 * k + 4,                   # xy, h_prev -> list, count
+        "knn_finish": 4 + 16 + 3 * 8 + 4 * k + 3 * 8 + rec,  # count, xy, pred -> list in; h, rho, c, record
+        "force": 4 * k + 80 + 64,                         # list, own state -> a, edot, v, e, x, y
     }[kernel]
+
+
+def profiled_traffic(precision: str, slot: str):
+    """HBM bytes per dispatch of a kernel from the committed rocprofv3
+    counter summary (profiles/hbm_traffic.txt: scaled FETCH_SIZE and
+    WRITE_SIZE of the clustered 4M case), or None."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 4 and f[0] == precision and f[1] == slot:
+            return int(float(f[2]) + float(f[3]))
+    return None
 
 
 def make_state(args):
@@ -342,18 +358,21 @@ def main():
                                   graph=args.graph, tune=args.tune)
         print(f"# fp64: {res['fp64']}", file=sys.stderr, flush=True)
     head = res.get("fp32c") or res["fp64"]
-    # dominant kernel and its roofline
+    # dominant single kernel and its roofline
     km = head["kernel_ms"]
-    dom = max(("knn_density", "force", "permute", "keys"), key=lambda n: km[n])
-    bytes_launch = algo_bytes(dom, k) * st["n"]
+    prec_name = "fp32c" if "fp32c" in res else "fp64"
+    cands = ("knn_walk", "knn_finish", "force", "permute", "keys")
+    dom = max(cands, key=lambda n: km[n])
+    bytes_launch = algo_bytes(dom, k, prec_name) * st["n"]
     ach = bytes_launch / (km[dom] * 1e-3) / 1e9
+    traffic = profiled_traffic(prec_name, dom) if args.workload == "clustered" else None
     roof = {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-            "algo_bytes_per_particle": algo_bytes(dom, k), "avg_ms": km[dom]}
-    df = ("knn_density", "force")
-    t_df = sum(km[n] for n in df)
-    b_df = sum(algo_bytes(n, k) for n in df) * st["n"]
-    roof["density_force_frac"] = round(b_df / (t_df * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "algo_bytes_per_particle": algo_bytes(dom, k, prec_name), "avg_ms": km[dom]}
+    df = ("knn_walk", "knn_finish", "force")
+    t_df = km["knn_density"] + km["force"]
+    b_df = sum(algo_bytes(n, k, prec_name) for n in df) * st["n"]
+    roof["knn_density_force_frac"] = round(b_df / (t_df * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
     line = {
         "metric": METRIC,
         "value": round(head["value"], 1),
