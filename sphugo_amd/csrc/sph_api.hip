@@ -174,9 +174,9 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
       (rc = h->alloc((char**)&pb.rec,
                      np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) ||
       (rc = h->alloc(&pb.partials, (int64_t)nitems * kNumRed)) ||
-      (rc = h->alloc(&pb.stats, 1)) || (rc = h->alloc(&pb.err, 1)) ||
+      (rc = h->alloc(&pb.stats, kStatSlots)) || (rc = h->alloc(&pb.err, 1)) ||
       (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
-      (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, np * 16)) ||
+      (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, np * kAnn)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
       (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
       (rc = h->alloc(&pb.sort.counts, sort_counts_size(np))) || (rc = h->alloc(&pb.sort.tot, 256)) ||
@@ -291,7 +291,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
       if (hipMemsetAsync(a, 0, np * sizeof(double), h->st) != hipSuccess)
         return bail(fail(SPH_EDEVICE, "memset failed"));
   }
-  if (hipMemsetAsync(pb.stats, 0, sizeof(KnnStats), h->st) != hipSuccess ||
+  if (hipMemsetAsync(pb.stats, 0, kStatSlots * sizeof(KnnStats), h->st) != hipSuccess ||
       hipMemsetAsync(pb.err, 0, sizeof(int), h->st) != hipSuccess ||
       hipMemsetAsync(pb.nbr, 0, nbr_entries(np, pb.kmax) * sizeof(uint32_t), h->st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "memset failed"));
@@ -365,10 +365,11 @@ static int run_pass(sph_sim* s, int perm_mode, int force_mode) {
 
 int sph::check_device_flags(sph_sim* s) {
   int err = 0;
-  KnnStats st;
+  KnnStats slots[kStatSlots];
   HIPCHK(hipMemcpyAsync(&err, s->pb.err, sizeof(int), hipMemcpyDeviceToHost, s->st));
-  HIPCHK(hipMemcpyAsync(&st, s->pb.stats, sizeof st, hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(slots, s->pb.stats, sizeof slots, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  const KnnStats st = sum_stats(slots);
   s->last_stats = st;
   if (err) return fail(SPH_ERANGE, "a particle left the periodic images [-L, 2L) of the box");
   if (st.failures) return fail(SPH_ERANGE, "kNN: %llu searches found fewer than k neighbours",
@@ -614,9 +615,10 @@ extern "C" int sph_set_timing(sph_sim* s, int32_t on) {
 
 extern "C" int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties) {
   if (!s) return fail(SPH_EINVAL, "null handle");
-  KnnStats st;
-  HIPCHK(hipMemcpyAsync(&st, s->pb.stats, sizeof st, hipMemcpyDeviceToHost, s->st));
+  KnnStats slots[kStatSlots];
+  HIPCHK(hipMemcpyAsync(slots, s->pb.stats, sizeof slots, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  const KnnStats st = sum_stats(slots);
   if (retries) *retries = (int64_t)st.retries;
   if (ties) *ties = (int64_t)st.ties;
   return 0;
@@ -624,9 +626,10 @@ extern "C" int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties) {
 
 extern "C" int sph_knn_counters(sph_sim* s, int64_t out[7]) {
   if (!s || !out) return fail(SPH_EINVAL, "null argument");
-  KnnStats st;
-  HIPCHK(hipMemcpyAsync(&st, s->pb.stats, sizeof st, hipMemcpyDeviceToHost, s->st));
+  KnnStats slots[kStatSlots];
+  HIPCHK(hipMemcpyAsync(slots, s->pb.stats, sizeof slots, hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  const KnnStats st = sum_stats(slots);
   const unsigned long long v[7] = {st.retries, st.ties, st.failures, st.miss_lo,
                                    st.miss_hi, st.miss_ann, st.split_fallback};
   for (int i = 0; i < 7; ++i) out[i] = (int64_t)v[i];

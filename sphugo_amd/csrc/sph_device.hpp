@@ -10,6 +10,7 @@ constexpr int kItem = 256;        // particles per work item (one block)
 constexpr int kMaxK = 64;         // neighbours per particle
 constexpr int kCandCap = 256;     // kNN candidates held per wave
 constexpr int kNumRed = 5;        // sum e, sum rho, E, Px, Py
+constexpr int kAnn = 16;          // kNN annulus entries kept per query
 
 // Go folds 6*40/(Pi*7), 4*7/(Pi*4), 8*7/(Pi*4) exactly and rounds once
 // (sim/sph.go:266,275,293,303); these are the correctly rounded values.
@@ -110,6 +111,17 @@ struct KnnStats {
   unsigned long long miss_ann;        // bracket: annulus over capacity
   unsigned long long split_fallback;  // tile too wide even for 8 lanes
 };
+
+// kNN counters are spread over kStatSlots copies, chosen by block, so that
+// thousands of waves do not serialise their atomics on one address; readers
+// sum the copies.
+constexpr int kStatSlots = 64;
+__device__ inline KnnStats* stat_slot(KnnStats* base) {
+  return base + (blockIdx.x & (kStatSlots - 1));
+}
+__device__ inline void stat_add(unsigned long long* p, unsigned long long v) {
+  if (v) atomicAdd(p, v);
+}
 
 // Work-item index for this block, arranged so that each of the 8 XCDs
 // (which receive blocks round-robin) walks a contiguous run of items and

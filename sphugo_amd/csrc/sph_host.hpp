@@ -60,7 +60,7 @@ struct PassBuffers {
   uint32_t* nbr;        // np * kmax packed (index << 4 | image code)
   void* rec;            // np force records (RecD or RecF)
   double* partials;     // nitems * kNumRed
-  KnnStats* stats;
+  KnnStats* stats;      // kStatSlots copies
   uint2* retry;           // kNN retry list (np entries)
   uint32_t* retry_count;
   uint32_t* knn_cnt;       // per query: certain | annulus << 8, or ~0 (retry)
@@ -73,6 +73,21 @@ struct PassBuffers {
 // Enqueue one force pass (keys, sort, permute, kNN + density, force,
 // reductions) on `st`.  Timing events (may be null) are recorded around
 // each kernel class.
+// Sum of the per-block copies of the kNN counters.
+inline KnnStats sum_stats(const KnnStats* v) {
+  KnnStats r = {};
+  for (int i = 0; i < kStatSlots; ++i) {
+    r.retries += v[i].retries;
+    r.ties += v[i].ties;
+    r.failures += v[i].failures;
+    r.miss_lo += v[i].miss_lo;
+    r.miss_hi += v[i].miss_hi;
+    r.miss_ann += v[i].miss_ann;
+    r.split_fallback += v[i].split_fallback;
+  }
+  return r;
+}
+
 // u32 entries of the neighbour lists of np queries (64-query tiles)
 inline int64_t nbr_entries(int64_t np, int kstride) { return (np + 63) / 64 * 64 * kstride; }
 

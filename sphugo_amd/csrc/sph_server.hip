@@ -103,7 +103,7 @@ struct sph_server {
   int* d_rc = nullptr;
   double* d_stage = nullptr;  // kNumRes rows of capacity
   int h_err = 0;
-  KnnStats h_stats = {};
+  KnnStats h_stats[kStatSlots] = {};
   unsigned long long seen_failures = 0;
   // slots freed and item lists changed at the end of a tick, uploaded at the
   // start of the next one (host buffers of a pending copy are not reused)
@@ -333,7 +333,7 @@ bool tick(sph_server* s) {
   if (!rc) {
     hipError_t e = hipMemcpyAsync(&s->h_err, c.pb.err, sizeof(int), hipMemcpyDeviceToHost, c.st);
     if (e == hipSuccess)
-      e = hipMemcpyAsync(&s->h_stats, c.pb.stats, sizeof(KnnStats), hipMemcpyDeviceToHost, c.st);
+      e = hipMemcpyAsync(s->h_stats, c.pb.stats, sizeof s->h_stats, hipMemcpyDeviceToHost, c.st);
     if (e == hipSuccess) e = hipStreamSynchronize(c.st);
     if (e != hipSuccess) rc = fail(SPH_EDEVICE, "tick: %s", hipGetErrorString(e));
   }
@@ -354,12 +354,13 @@ bool tick(sph_server* s) {
   s->ticks += 1;
   s->occupancy_sum += (double)occupied / (double)s->cap;
   s->tick_ms_sum += ms_between(t_tick, Clock::now());
-  const bool bad = s->h_err != 0 || s->h_stats.failures != s->seen_failures;
+  const unsigned long long failures = sum_stats(s->h_stats).failures;
+  const bool bad = s->h_err != 0 || failures != s->seen_failures;
   if (bad) {
     // the flags are pool-wide: every job of this pass is failed
     fail_all_running(s, s->h_err ? "a particle left the periodic images [-L, 2L) of the box"
                                  : "kNN found fewer than k neighbours");
-    s->seen_failures = s->h_stats.failures;
+    s->seen_failures = failures;
     if (s->h_err && hipMemsetAsync(c.pb.err, 0, sizeof(int), c.st) != hipSuccess) {
       s->fatal = SPH_EDEVICE;
       s->fatal_msg = "could not clear the error flag";
@@ -467,7 +468,7 @@ extern "C" int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kma
   pb.nitems = 0;
   pb.nitems_all = 0;
   if (hipMemsetAsync(d_sims, 0, max_jobs * sizeof(SimDev), c.st) != hipSuccess ||
-      hipMemsetAsync(pb.stats, 0, sizeof(KnnStats), c.st) != hipSuccess ||
+      hipMemsetAsync(pb.stats, 0, kStatSlots * sizeof(KnnStats), c.st) != hipSuccess ||
       hipMemsetAsync(pb.err, 0, sizeof(int), c.st) != hipSuccess ||
       hipMemsetAsync(pb.nbr, 0, nbr_entries(capacity, pb.kmax) * sizeof(uint32_t), c.st) != hipSuccess ||
       hipStreamSynchronize(c.st) != hipSuccess)

@@ -168,8 +168,7 @@ __device__ inline void cell_range(const SimDev* S, const uint32_t* __restrict__ 
 
 constexpr int kV1Cells = 128;  // coarse cells of one search (<= 9 x 9 used)
 constexpr int kTileCells = 256;  // coarse cells of one tile box (<= 15 x 15 used)
-constexpr int kAnn = 12;       // annulus entries kept per query
-constexpr double kDelta = 0.02;  // bracket half-width around the last h
+constexpr double kDelta = 0.03;  // bracket half-width around the last h
 constexpr int kSplit = 24;       // max tile candidates per query before a split
 constexpr int kSub = 8;          // sub-tiles of a wave (8 queries each)
 constexpr int kSubW = 64 / kSub;
@@ -640,7 +639,7 @@ __global__ __launch_bounds__(256) void k_knn(
     // Lane groups: aligned blocks of consecutive queries.  A block whose
     // box holds more than kSplit candidates per query is halved (Morton
     // order can put two distant regions in one wave); blocks below 8 lanes
-    // go to the wavefront search instead.
+    // go to the wavefront search, which handles them at lower cost.
     unsigned long long pending = ballot(fast);
     int gsz = 64;
     while (pending) {
@@ -872,7 +871,8 @@ __global__ __launch_bounds__(256) void k_knn(
       if (lane == 0) base = atomicAdd(retry_count, (uint32_t)__popcll(m));
       base = readlane_u(base, 0);
       if (need) retry[base + lane_prefix(m)] = make_uint2((uint32_t)t, (uint32_t)item_idx);
-      if (lane == 0) nfall += __popcll(m & ballot(hp > 0));
+      const unsigned long long mh = m & ballot(hp > 0);
+      if (lane == 0) nfall += __popcll(mh);
     }
   }
   nretry += nfall;
@@ -885,14 +885,15 @@ __global__ __launch_bounds__(256) void k_knn(
     nmiss_ann += __shfl_xor(nmiss_ann, o, 64);
     nsplitfb += __shfl_xor(nsplitfb, o, 64);
   }
-  if (lane == 0 && (nretry | nties | nfail | nsplitfb)) {
-    atomicAdd(&stats->retries, nretry);
-    atomicAdd(&stats->ties, nties);
-    atomicAdd(&stats->failures, nfail);
-    atomicAdd(&stats->miss_lo, nmiss_lo);
-    atomicAdd(&stats->miss_hi, nmiss_hi);
-    atomicAdd(&stats->miss_ann, nmiss_ann);
-    atomicAdd(&stats->split_fallback, nsplitfb);
+  if (lane == 0) {
+    KnnStats* sp = stat_slot(stats);
+    stat_add(&sp->retries, nretry);
+    stat_add(&sp->ties, nties);
+    stat_add(&sp->failures, nfail);
+    stat_add(&sp->miss_lo, nmiss_lo);
+    stat_add(&sp->miss_hi, nmiss_hi);
+    stat_add(&sp->miss_ann, nmiss_ann);
+    stat_add(&sp->split_fallback, nsplitfb);
   }
 }
 
@@ -1029,7 +1030,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     write_result<FP32C>(S, t, xq, yq, tau, facc, EP, VPX, VPY, H, RHO, C, recp);
   }
   for (int o = 32; o > 0; o >>= 1) nties += __shfl_xor(nties, o, 64);
-  if (lane == 0 && nties) atomicAdd(&stats->ties, nties);
+  if (lane == 0) stat_add(&stat_slot(stats)->ties, nties);
 }
 
 // Exact wavefront search for the queries of the retry list (or, in
@@ -1074,10 +1075,11 @@ __global__ __launch_bounds__(256) void k_knn_fallback(
     if (lane == 0 && so.tau > 0)
       write_result<FP32C>(S, t, xq, yq, so.tau, so.facc, EP, VPX, VPY, H, RHO, C, recp);
   }
-  if (lane == 0 && (nretry | nties | nfail)) {
-    atomicAdd(&stats->retries, nretry);
-    atomicAdd(&stats->ties, nties);
-    atomicAdd(&stats->failures, nfail);
+  if (lane == 0) {
+    KnnStats* sp = stat_slot(stats);
+    stat_add(&sp->retries, nretry);
+    stat_add(&sp->ties, nties);
+    stat_add(&sp->failures, nfail);
   }
 }
 
