@@ -61,7 +61,23 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_demo()
     return LIB
+
+
+DEMO = os.path.join(HERE, "sim_demo")
+
+
+def build_demo() -> str:
+    """The C++ mirror of package `sim` (csrc/sim.hpp) is header-only over the
+    C ABI; sim_demo exercises it (host code only, g++)."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-I", CSRC,
+           os.path.join(CSRC, "sim_demo.cpp"), "-o", DEMO, "-L", HERE, "-lsphugo_amd",
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed on sim_demo.cpp:\n{r.stderr}")
+    return DEMO
 
 
 if __name__ == "__main__":
