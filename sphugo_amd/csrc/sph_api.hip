@@ -146,7 +146,7 @@ static int64_t layout_cells(sph_sim* h, double cpp) {
     coff += (int64_t)h->hsims[s].gx * h->hsims[s].gy;
   }
   int bits = 1;
-  while ((1LL << bits) <= coff) ++bits;
+  while ((1LL << bits) < coff) ++bits;  // keys lie in [0, coff)
   h->pb.ncells = coff;
   h->pb.key_bits = bits;
   h->grid_cpp = cpp;

@@ -193,7 +193,7 @@ int upload_items(sph_server* s) {
   pb.nitems_all = (int)s->h_items_all.size();
   pb.np = hi;
   int bits = 1;
-  while ((1LL << bits) <= hi * (int64_t)kHoleCpp) ++bits;
+  while ((1LL << bits) < hi * (int64_t)kHoleCpp) ++bits;  // keys lie in [0, 16 hi)
   pb.key_bits = bits;
   const hipStream_t st = s->core.st;
   HIPCHK(hipMemcpyAsync(s->d_items, s->h_items.data(), s->h_items.size() * sizeof(Item),
@@ -448,7 +448,7 @@ extern "C" int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kma
   pb.np = capacity;
   pb.ncells = capacity * (int64_t)kHoleCpp;
   int bits = 1;
-  while ((1LL << bits) <= pb.ncells) ++bits;
+  while ((1LL << bits) < pb.ncells) ++bits;
   pb.key_bits = bits;
   pb.hole_cpp = kHoleCpp;
   pb.have_h = c.bitwise ? 0 : 1;  // jobs without h take the exact search

@@ -636,6 +636,7 @@ __global__ __launch_bounds__(256) void k_knn(
     u32x4 cbuf = {0u, 0u, 0u, 0u};
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const uint32_t tu = (uint32_t)t;
+    const uint32_t selfpk = (tu << 4) | img_code(0, 0);
     // Lane groups: aligned blocks of consecutive queries.  A block whose
     // box holds more than kSplit candidates per query is halved (Morton
     // order can put two distant regions in one wave); blocks below 8 lanes
@@ -787,21 +788,20 @@ __global__ __launch_bounds__(256) void k_knn(
         __builtin_amdgcn_wave_barrier();
         nm = (__builtin_amdgcn_readfirstlane(nm) + 3) & ~3;
         const uint8_t* mylist = ts.slist[my_sub];
-#define SPH_ACCEPT(D2, PK)                                                              \
+// Certain neighbours shift through a 4-entry register buffer (oldest in x)
+// that leaves as one 16-byte chunk store every fourth entry.
+#define SPH_ACCEPT(D2, PK, NOTSELF)                                                     \
   {                                                                                     \
     const double d2v_ = (D2);                                                           \
     const uint32_t pkv_ = (PK);                                                         \
-    const bool ok_ = grp && d2v_ <= Rhi2 && (pkv_ >> 4) != tu;                          \
+    const bool ok_ = grp && d2v_ <= Rhi2 && (NOTSELF);                                  \
     const bool isc_ = ok_ && d2v_ < Rlo2;                                               \
     const bool isa_ = ok_ && !(d2v_ < Rlo2);                                            \
-    if (isc_) {                                                                         \
-      const int sl_ = c_lo & 3;                                                         \
-      cbuf.x = sl_ == 0 ? pkv_ : cbuf.x;                                                \
-      cbuf.y = sl_ == 1 ? pkv_ : cbuf.y;                                                \
-      cbuf.z = sl_ == 2 ? pkv_ : cbuf.z;                                                \
-      cbuf.w = sl_ == 3 ? pkv_ : cbuf.w;                                                \
-      if (sl_ == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;                                  \
-    }                                                                                   \
+    cbuf.x = isc_ ? cbuf.y : cbuf.x;                                                    \
+    cbuf.y = isc_ ? cbuf.z : cbuf.y;                                                    \
+    cbuf.z = isc_ ? cbuf.w : cbuf.z;                                                    \
+    cbuf.w = isc_ ? pkv_ : cbuf.w;                                                      \
+    if (isc_ && (c_lo & 3) == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;              \
     if (isa_ && c_an < kAnn) ts.ann_pk[c_an][lane] = pkv_;                              \
     c_lo += isc_ ? 1 : 0;                                                               \
     c_an += isa_ ? 1 : 0;                                                               \
@@ -818,10 +818,11 @@ __global__ __launch_bounds__(256) void k_knn(
             double a3 = xq - e3.x, b3 = yq - e3.y;
             const double d0 = a0 * a0 + b0 * b0, d1 = a1 * a1 + b1 * b1;
             const double d2_ = a2 * a2 + b2 * b2, d3 = a3 * a3 + b3 * b3;
-            SPH_ACCEPT(d0, p4.x);
-            SPH_ACCEPT(d1, p4.y);
-            SPH_ACCEPT(d2_, p4.z);
-            SPH_ACCEPT(d3, p4.w);
+            // interior candidates: only the query itself at image 0 is self
+            SPH_ACCEPT(d0, p4.x, p4.x != selfpk);
+            SPH_ACCEPT(d1, p4.y, p4.y != selfpk);
+            SPH_ACCEPT(d2_, p4.z, p4.z != selfpk);
+            SPH_ACCEPT(d3, p4.w, p4.w != selfpk);
           }
         } else {
           for (int m = 0; m < nm; ++m) {
@@ -833,7 +834,7 @@ __global__ __launch_bounds__(256) void k_knn(
             const double2 e = ts.sxy[im];
             const double ex = ax - e.x;
             const double ey = ay - e.y;
-            SPH_ACCEPT(ex * ex + ey * ey, pk);
+            SPH_ACCEPT(ex * ex + ey * ey, pk, (pk >> 4) != tu);
           }
         }
 #undef SPH_ACCEPT
@@ -850,11 +851,20 @@ __global__ __launch_bounds__(256) void k_knn(
       else if (c_lo + c_an < k) ++nmiss_hi;
     }
     if (fast && c_lo < k && c_an <= kAnn && c_lo + c_an >= k) {
-      if (c_lo & 3) {
+      // the last c_lo % 4 certain entries sit at the top of the buffer
+      const int rem = c_lo & 3;
+      if (rem) {
         uint32_t* r0 = &row[c_lo & ~3];
-        r0[0] = cbuf.x;
-        if ((c_lo & 3) > 1) r0[1] = cbuf.y;
-        if ((c_lo & 3) > 2) r0[2] = cbuf.z;
+        if (rem == 1) {
+          r0[0] = cbuf.w;
+        } else if (rem == 2) {
+          r0[0] = cbuf.z;
+          r0[1] = cbuf.w;
+        } else {
+          r0[0] = cbuf.y;
+          r0[1] = cbuf.z;
+          r0[2] = cbuf.w;
+        }
       }
       uint32_t* ao = ann + t * kAnn;
       for (int e = 0; e < c_an; ++e) ao[e] = ts.ann_pk[e][lane];

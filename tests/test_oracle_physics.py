@@ -195,3 +195,46 @@ def test_cfl_dt_definition(oracle_lib):
     h, c = s.get("h"), s.get("c")
     v = np.hypot(s.get("vx"), s.get("vy"))
     assert s.cfl_dt_half() == 0.5 * (0.3 * np.min((0.5 * h) / (c + v)))
+
+
+def _moving_particle(O, u, alpha, beta, i=137):
+    """Square lattice at rest except particle i moving with vx = u; k = 20
+    fills whole shells (r^2 = 1, 2, 4, 5 spacings^2)."""
+    x, y = square_lattice(16)
+    vx = np.zeros(len(x))
+    vx[i] = u
+    s = _sim(O, x, y, vx=vx, k=20, visc_alpha=alpha, visc_beta=beta)
+    return s.get("ax")[i], s.get("ay")[i], s.get("edot")[i], s
+
+
+@pytest.mark.parametrize("kernel", [MON])
+def test_viscosity_structure(oracle_lib, kernel):
+    """Viscosity (sim/sph.go:370-390) acts only on approaching pairs (dot < 0),
+    is linear in the relative velocity through alpha and quadratic through
+    beta, and the pressure terms and EDot cancel over the symmetric lattice."""
+    u = 0.3
+    # no viscosity: a lattice point feels no net pressure force
+    ax0, ay0, ed0, _ = _moving_particle(oracle_lib, u, 0.0, 0.0)
+    assert abs(ax0) < 1e-10 and abs(ay0) < 1e-10
+    # EDot = A m sum dot DF: dot = -u (x_j - x_i) is odd over the lattice shells
+    assert abs(ed0) < 1e-12
+    # alpha only: linear in u, decelerating, mirror-odd
+    a1, ay1, ed1, _ = _moving_particle(oracle_lib, u, 1.0, 0.0)
+    a2, _, _, _ = _moving_particle(oracle_lib, 2 * u, 1.0, 0.0)
+    am, _, _, _ = _moving_particle(oracle_lib, -u, 1.0, 0.0)
+    assert a1 < 0 and abs(a1) > 1e3 * abs(ax0)
+    assert abs(a2 - 2 * a1) <= 1e-12 * abs(a1)
+    assert abs(am + a1) <= 1e-12 * abs(a1)
+    assert abs(ay1) <= 1e-12 * abs(a1)
+    # the viscous EDot is not in the reference's EDot (quirk Q6): still zero
+    assert abs(ed1) < 1e-12
+    # beta only: quadratic in u, still decelerating
+    b1, _, _, _ = _moving_particle(oracle_lib, u, 0.0, 1.0)
+    b2, _, _, _ = _moving_particle(oracle_lib, 2 * u, 0.0, 1.0)
+    assert b1 < 0 and abs(b2 - 4 * b1) <= 1e-12 * abs(b2)
+    # both: the sum of the two parts (the bracket is linear in Pi)
+    ab, _, _, _ = _moving_particle(oracle_lib, u, 1.0, 1.0)
+    assert abs(ab - (a1 + b1)) <= 1e-12 * abs(ab)
+    # alpha scales the linear part exactly
+    a3, _, _, _ = _moving_particle(oracle_lib, u, 3.0, 0.0)
+    assert abs(a3 - 3 * a1) <= 1e-12 * abs(a3)
