@@ -250,7 +250,8 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
   const int lane = lane_id();
   const int k = S->k;
   const double lx = S->lx, ly = S->ly;
-  const double rcap = 1.5 * (lx > ly ? lx : ly);
+  // every image of every particle lies within 2 sqrt(2) max(lx, ly) of the query
+  const double rcap = 3.0 * (lx > ly ? lx : ly);
   if (R > rcap) R = rcap;
   double r_lo = 0.0, r_hi = 0.0;
   int cnt = 0;

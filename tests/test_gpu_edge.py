@@ -1,0 +1,141 @@
+"""Edge cases of the HIP path against the oracle (GPU): gravity and
+reflection walls, the Wendland kernel over several steps, anisotropic and
+shifted boxes, exact distance ties on lattices, the smallest k and N, and a
+batch whose simulations have different boxes."""
+import numpy as np
+import pytest
+
+import oracle as O
+from sphugo_amd import _lib as L
+from sphugo_amd import ics
+
+from test_gpu_parity import _check_forces, _check_knn
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(st, k, hor=(0, 1), vert=(0, 1), precision=L.PREC_FP64, flags=0, kernel=0,
+          forces=True, **kw):
+    kw.setdefault("gamma", 5 / 3)
+    kw.setdefault("visc_alpha", 1.0)
+    kw.setdefault("visc_beta", 2.0)
+    common = dict(k=k, kernel=kernel, mass=st["mass"], hor=hor, vert=vert, **kw)
+    g = L.Sim(L.default_params(precision=precision, flags=flags, **common), st["x"], st["y"],
+              st["vx"], st["vy"], st["e"])
+    use_cfl = 1 if common.get("dt_half", 0.001) <= 0 else 0
+    o = O.OracleSim(O.default_params(knn_mode=O.KNN_EXACT, use_cfl=use_cfl, **common),
+                    st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    if forces:
+        for s in (g, o):
+            s.set("vpx", st["vx"])
+            s.set("vpy", st["vy"])
+            s.set("epred", st["e"])
+            s.calculate_forces()
+    return g, o
+
+
+def _lattice(n, jitter=0.0):
+    gl = (np.arange(n) + 0.5) / n
+    x, y = [a.ravel().copy() for a in np.meshgrid(gl, gl, indexing="ij")]
+    return {"x": x, "y": y, "vx": np.zeros(n * n), "vy": np.zeros(n * n), "e": np.ones(n * n),
+            "mass": 1 / (n * n)}
+
+
+def test_gravity_and_walls_bitwise():
+    st = ics.blast(3000, seed=21)
+    g, o = _pair(st, 32, flags=L.FLAG_BITWISE, forces=False, dt_half=0.0, cfl=0.3,
+                 accel=(0.0, -2.0), refl=(0.05, 0.95, 0.05, 0.95))
+    g.step(6)
+    o.step(6)
+    for f in L.FIELDS:
+        assert (g.get(f) == o.get(f)).all(), f
+
+
+def test_wendland_steps_bitwise():
+    st = ics.clustered(4000, seed=5, k=32)
+    g, o = _pair(st, 32, kernel=1, flags=L.FLAG_BITWISE, forces=False, dt_half=0.0, cfl=0.3)
+    g.step(4)
+    o.step(4)
+    for f in L.FIELDS:
+        assert (g.get(f) == o.get(f)).all(), f
+
+
+@pytest.mark.parametrize("hor,vert", [((-2.0, 3.0), (1.0, 1.5)), ((0.25, 0.5), (-4.0, 4.0))])
+def test_anisotropic_box(hor, vert):
+    st = ics.uniform(3000, seed=7)
+    st = dict(st)
+    st["x"] = hor[0] + (hor[1] - hor[0]) * st["x"]
+    st["y"] = vert[0] + (vert[1] - vert[0]) * st["y"]
+    g, o = _pair(st, 32, hor=hor, vert=vert)
+    _check_knn(g, o)
+    _check_forces(g, o)
+    g, o = _pair(st, 32, hor=hor, vert=vert, flags=L.FLAG_BITWISE, forces=False, dt_half=0.0,
+                 cfl=0.3)
+    g.step(3)
+    o.step(3)
+    for f in L.FIELDS:
+        assert (g.get(f) == o.get(f)).all(), f
+
+
+@pytest.mark.parametrize("k", [32, 40, 64])
+@pytest.mark.parametrize("mode", ["fp64", "bitwise", "fp32c"])
+def test_lattice_ties(k, mode):
+    # a square lattice has exact squared-distance ties at every shell; these
+    # k cut a shell, so the k-th neighbours are chosen by (d2, ID, image)
+    st = _lattice(24)
+    prec = L.PREC_FP32C if mode == "fp32c" else L.PREC_FP64
+    flags = L.FLAG_BITWISE if mode == "bitwise" else 0
+    g, o = _pair(st, k, precision=prec, flags=flags)
+    _check_knn(g, o)
+    if mode != "fp32c":
+        # on a lattice the pressure terms cancel: a is rounding noise, held
+        # against the pressure-gradient scale c^2 / h
+        for f in ("rho", "c"):
+            d = np.abs(g.get(f) - o.get(f)) / o.get(f)
+            assert d.max() <= (0 if mode == "bitwise" else 1e-12), f
+        scale = (o.get("c") ** 2 / o.get("h")).max()
+        for f in ("ax", "ay", "edot"):
+            d = np.abs(g.get(f) - o.get(f)).max()
+            assert d <= (0 if mode == "bitwise" else 1e-12 * scale), f
+    # a second pass starts from the previous h (the bracketed path)
+    g.calculate_forces()
+    o.calculate_forces()
+    _check_knn(g, o)
+
+
+def test_smallest_k_and_n():
+    st = ics.uniform(40, seed=3)
+    g, o = _pair(st, 2)
+    _check_knn(g, o)
+    _check_forces(g, o)
+    # 9 (n - 1) >= k: five particles give 36 candidates over the 9 images,
+    # all of them neighbours (h up to 2.2 box lengths)
+    for n, k in ((5, 36), (10, 64)):
+        st = ics.uniform(n, seed=4)
+        g, o = _pair(st, k)
+        _check_knn(g, o)
+        _check_forces(g, o)
+
+
+def test_batch_with_different_boxes():
+    boxes = [((0, 1), (0, 1)), ((-1, 2), (0, 0.5)), ((10, 11), (10, 13))]
+    sts, ps = [], []
+    for i, (hor, vert) in enumerate(boxes):
+        st = dict(ics.blast(1500 + 700 * i, seed=31 + i))
+        st["x"] = hor[0] + (hor[1] - hor[0]) * st["x"]
+        st["y"] = vert[0] + (vert[1] - vert[0]) * st["y"]
+        sts.append(st)
+        ps.append(L.default_params(k=32, mass=st["mass"], gamma=5 / 3, hor=hor, vert=vert,
+                                   dt_half=0.0, cfl=0.3, visc_alpha=1.0, visc_beta=2.0,
+                                   flags=L.FLAG_BITWISE))
+    b = L.Batch(ps, sts)
+    b.step(3)
+    for i, (hor, vert) in enumerate(boxes):
+        st = sts[i]
+        o = O.OracleSim(O.default_params(knn_mode=O.KNN_EXACT, use_cfl=1, cfl=0.3, k=32,
+                                         mass=st["mass"], gamma=5 / 3, hor=hor, vert=vert,
+                                         visc_alpha=1.0, visc_beta=2.0),
+                        st["x"], st["y"], st["vx"], st["vy"], st["e"])
+        o.step(3)
+        for f in ("x", "y", "vx", "vy", "e", "rho", "h"):
+            assert (b.get_sim(i, f) == o.get(f)).all(), (i, f)
