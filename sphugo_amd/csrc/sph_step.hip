@@ -1191,14 +1191,17 @@ __global__ __launch_bounds__(256) void k_force(
     const float alpha = (float)S->alpha, beta = (float)S->beta, eta2 = (float)S->eta2;
     // Kahan-compensated sums
     float fx = 0.f, fy = 0.f, fe = 0.f, cx = 0.f, cy = 0.f, ce = 0.f;
-    for (int n4 = 0; n4 < k; n4 += 4) {
+    // 8 records in flight per lane (48 B each): more memory-level
+    // parallelism pays more than the occupancy it costs
+    for (int n4 = 0; n4 < k; n4 += 8) {
       const u32x4 q4 = __builtin_nontemporal_load(row.chunk(n4 >> 2));
-      const uint32_t pks[4] = {q4.x, q4.y, q4.z, q4.w};
-      RecF nbs[4];
+      const u32x4 q5 = n4 + 4 < k ? __builtin_nontemporal_load(row.chunk((n4 >> 2) + 1)) : q4;
+      const uint32_t pks[8] = {q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
+      RecF nbs[8];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) nbs[u] = rec[n4 + u < k ? (pks[u] >> 4) : (uint32_t)t];
+      for (int u = 0; u < 8; ++u) nbs[u] = rec[n4 + u < k ? (pks[u] >> 4) : (uint32_t)t];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
+      for (int u = 0; u < 8; ++u) {
         if (n4 + u >= k) break;
         const uint32_t pk = pks[u];
         const RecF& nb = nbs[u];
