@@ -88,10 +88,12 @@ __device__ inline uint32_t img_order(uint32_t code) {
 struct alignas(16) RecD {
   double x, y, vpx, vpy, rho, c, h, B;  // B = c^2 / (gamma rho)
 };
+// FP32C record: the position as 32-bit fixed point of the box (units of
+// L / 2^32, wrapped modulo 2^32), so that the 32-bit difference of two
+// records is the minimum-image displacement; the rest in FP32.
 struct alignas(16) RecF {
-  double x, y;
-  float vpx, vpy, rho, c;
-  float h, B, pad0, pad1;
+  uint32_t ux, uy;
+  float vpx, vpy, rho, c, h, B;
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));

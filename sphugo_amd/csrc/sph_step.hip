@@ -540,16 +540,14 @@ __device__ inline void write_result(const SimDev* S, int64_t t, double xq, doubl
   const double B = c * c / (S->gamma * rho);
   if constexpr (FP32C) {
     RecF r;
-    r.x = xq;
-    r.y = yq;
+    r.ux = (uint32_t)(int64_t)floor((xq - S->x0) * (4294967296.0 / S->lx));
+    r.uy = (uint32_t)(int64_t)floor((yq - S->y0) * (4294967296.0 / S->ly));
     r.vpx = (float)VPX[t];
     r.vpy = (float)VPY[t];
     r.rho = (float)rho;
     r.c = (float)c;
     r.h = (float)h;
     r.B = (float)B;
-    r.pad0 = 0.f;
-    r.pad1 = 0.f;
     ((RecF*)recp)[t] = r;
   } else {
     RecD r;
@@ -1102,7 +1100,7 @@ template <bool FP32C, bool EXACT>
 __global__ __launch_bounds__(256) void k_force(
     const Item* __restrict__ items, int nitems, SimDev* __restrict__ sims,
     const void* __restrict__ recp, Fields f, const uint32_t* __restrict__ nbr, int kstride,
-    int mode, Positions pout, double* __restrict__ partials) {
+    int mode, Positions pout, double* __restrict__ partials, const double2* __restrict__ XY) {
   __shared__ double red[4][kNumRed + 1];
   const int item_idx = xcd_item(nitems);
   const Item it = items[item_idx];
@@ -1181,8 +1179,13 @@ __global__ __launch_bounds__(256) void k_force(
   } else {
     const RecF* rec = (const RecF*)recp;
     const RecF me = rec[t];
-    xi = me.x; yi = me.y;
+    const double2 pme = XY[t];
+    xi = pme.x; yi = pme.y;
     hi = (double)me.h;
+    // displacements from the fixed-point difference (the minimum image) hold
+    // while every neighbour lies within half a box; otherwise from FP64
+    const bool fixed = hi < 0.45 * (lx < ly ? lx : ly);
+    const float fsx = (float)(lx * 0x1p-32), fsy = (float)(ly * 0x1p-32);
     ci = (double)me.c;
     rhoi = (double)me.rho;
     A = (double)me.B;
@@ -1205,8 +1208,16 @@ __global__ __launch_bounds__(256) void k_force(
         if (n4 + u >= k) break;
         const uint32_t pk = pks[u];
         const RecF& nb = nbs[u];
-        const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-        const float rx = (float)((nb.x - ox) - xi), ry = (float)((nb.y - oy) - yi);
+        float rx, ry;
+        if (fixed) {
+          rx = (float)(int32_t)(nb.ux - me.ux) * fsx;
+          ry = (float)(int32_t)(nb.uy - me.uy) * fsy;
+        } else {
+          const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+          const double2 pj = XY[pk >> 4];
+          rx = (float)((pj.x - ox) - xi);
+          ry = (float)((pj.y - oy) - yi);
+        }
         const float d2 = rx * rx + ry * ry;
         const float dinv = __builtin_amdgcn_rsqf(d2);
         float q = (d2 * dinv) * hinv;
@@ -1431,7 +1442,8 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   mark(4, 0);
 #define SPH_FORCE(F32, EX)                                                                     \
   hipLaunchKernelGGL((k_force<F32, EX>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,     \
-                     (const void*)b.rec, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur, b.partials)
+                     (const void*)b.rec, b.fnew, b.nbr, b.kmax, force_mode, b.pos_cur, b.partials, \
+                     b.xyp)
   if (b.prec == kPrecFP32C) SPH_FORCE(true, false);
   else if (b.bitwise) SPH_FORCE(false, true);
   else SPH_FORCE(false, false);
