@@ -91,23 +91,43 @@ __global__ __launch_bounds__(kItem) void k_permute(
     Positions pin, Fields fin, double2* __restrict__ xyp, Fields fout, uint8_t* __restrict__ wrap,
     uint32_t* __restrict__ cstart) {
   const Item it = items[xcd_item(nitems)];
-  if ((int)threadIdx.x >= it.count) return;
   if (it.sim < 0) return;  // unused pool slot: stays in place, owns no cells
-  const int64_t t = it.first + threadIdx.x;
-  const uint32_t i = perm[t];
   const SimDev* S = sims + it.sim;
-  // cell ranges of this simulation's cells [coff, coff + G^2]: cells
-  // (key[t-1], key[t]] start at t; the last particle closes the range.
+  const int lane = threadIdx.x & 63;
+  const int w0 = threadIdx.x & ~63;  // first query of this wave in the item
+  if (w0 >= it.count) return;
+  const bool valid = (int)threadIdx.x < it.count;
+  const int64_t t = it.first + (valid ? threadIdx.x : it.count - 1);
+  // Cell ranges of this simulation's cells [coff, coff + G^2]: cell q starts
+  // at the first particle whose key is >= q; the simulation's last particle
+  // closes the range.  The wave writes the contiguous block of cells
+  // (key[t0 - 1], key[t_last]] of its particles together, coalesced, each
+  // cell finding its particle by a binary search over the lanes' keys.
   // Cells outside every simulation's range are never read.
   {
-    const int64_t kt = skeys[t];
-    const int64_t lo = t == S->off ? S->coff : (int64_t)skeys[t - 1] + 1;
-    for (int64_t q = lo; q <= kt; ++q) cstart[q] = (uint32_t)t;
-    if (t == S->off + S->n - 1) {
+    const int64_t t0 = it.first + w0;
+    const int nv = (it.count - w0) < 64 ? (it.count - w0) : 64;
+    const uint32_t kt = skeys[t];
+    const int64_t lo = (t0 == S->off ? S->coff : (int64_t)skeys[t0 - 1] + 1);
+    const int64_t hi = (int64_t)__shfl(kt, nv - 1, 64);
+    for (int64_t qb = lo; qb <= hi; qb += 64) {  // uniform: every lane shuffles
+      const int64_t q = qb + lane;
+      int a = 0, b = nv - 1;  // smallest lane with key >= q (lane nv-1 qualifies)
+      for (int st = 0; st < 6; ++st) {
+        const int mid = (a + b) >> 1;
+        const int64_t km = (int64_t)__shfl(kt, mid, 64);
+        if (km >= q) b = mid; else a = mid + 1;
+      }
+      if (q <= hi) cstart[q] = (uint32_t)(t0 + a);
+    }
+    const int64_t tl = t0 + nv - 1;
+    if (tl == S->off + S->n - 1) {
       const int64_t end = S->coff + (int64_t)S->gx * S->gy;
-      for (int64_t q = kt + 1; q <= end; ++q) cstart[q] = (uint32_t)(t + 1);
+      for (int64_t q = hi + 1 + lane; q <= end; q += 64) cstart[q] = (uint32_t)(tl + 1);
     }
   }
+  if (!valid) return;
+  const uint32_t i = perm[t];
   if (mode < 0) mode = S->pmode;
   double x = pin.x[i], y = pin.y[i];
   const double vx = fin.vx[i], vy = fin.vy[i], e = fin.e[i];
