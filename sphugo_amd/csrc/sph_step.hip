@@ -1065,8 +1065,11 @@ __global__ __launch_bounds__(256) void k_knn_finish(
 // Exact wavefront search for the queries of the retry list (or, in
 // all-queries mode, for every particle: first pass and bitwise order).
 // Waves stride over the list, so the slow searches spread over the GPU.
+// Latency-bound (a chain of dependent loads per query): held to four waves
+// per SIMD (a few bytes spill) it hides that latency better than at the
+// three its registers would otherwise allow.
 template <bool BITWISE, bool FP32C>
-__global__ __launch_bounds__(256) void k_knn_fallback(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_knn_fallback(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const double2* __restrict__ XY,
     const uint8_t* __restrict__ WR, const uint32_t* __restrict__ cstart,
