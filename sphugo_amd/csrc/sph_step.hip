@@ -216,7 +216,6 @@ struct TileShared {
   uint32_t spk[65];
   double sbb[kSub][4];       // box of each sub-tile (x0, x1, y0, y1)
   uint8_t slist[kSub][64];   // staged candidates within reach of each sub-tile
-  uint32_t ann_pk[kAnn][64];
 };
 
 union WaveShared {
@@ -610,11 +609,15 @@ __device__ inline double local_radius(const SimDev* S, const uint32_t* __restric
 // radius) through LDS 64 at a time and every lane tests all of them against
 // its bracket [h_prev (1 - delta), h_prev (1 + delta)]: nearer candidates
 // are certain neighbours and go straight to the list; the annulus keeps at
-// most kAnn entries from which the remaining ones are selected exactly.
+// most kAnn entries (written straight to global memory, which keeps the
+// wave's LDS small) from which the remaining ones are selected exactly.
 // Queries without a previous h, or whose bracket misses, fall back to the
 // exact wavefront search.  BITWISE uses the wavefront search for all.
+// Six waves per SIMD: the walk is issue-bound with long LDS and gather
+// latencies, and the extra waves (at the cost of a few spilled registers)
+// hide more of them than the registers they take away.
 template <bool BITWISE, bool FP32C>
-__global__ __launch_bounds__(256) void k_knn(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_knn(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const double2* __restrict__ XY,
     const uint8_t* __restrict__ WR, const uint32_t* __restrict__ cstart,
@@ -656,6 +659,7 @@ __global__ __launch_bounds__(256) void k_knn(
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const uint32_t tu = (uint32_t)t;
     const uint32_t selfpk = (tu << 4) | img_code(0, 0);
+    uint32_t* annrow = ann + t * kAnn;  // annulus candidates go straight out
     // Lane groups: aligned blocks of consecutive queries.  A block whose
     // box holds more than kSplit candidates per query is halved (Morton
     // order can put two distant regions in one wave); blocks below 8 lanes
@@ -821,7 +825,7 @@ __global__ __launch_bounds__(256) void k_knn(
     cbuf.z = isc_ ? cbuf.w : cbuf.z;                                                    \
     cbuf.w = isc_ ? pkv_ : cbuf.w;                                                      \
     if (isc_ && (c_lo & 3) == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;              \
-    if (isa_ && c_an < kAnn) ts.ann_pk[c_an][lane] = pkv_;                              \
+    if (isa_ && c_an < kAnn) annrow[c_an] = pkv_;                                       \
     c_lo += isc_ ? 1 : 0;                                                               \
     c_an += isa_ ? 1 : 0;                                                               \
   }
@@ -885,8 +889,6 @@ __global__ __launch_bounds__(256) void k_knn(
           r0[2] = cbuf.w;
         }
       }
-      uint32_t* ao = ann + t * kAnn;
-      for (int e = 0; e < c_an; ++e) ao[e] = ts.ann_pk[e][lane];
       resolved = true;
     }
     if (valid) cnt[t] = resolved ? ((uint32_t)c_lo | ((uint32_t)c_an << 8)) : 0xFFFFFFFFu;
