@@ -7,8 +7,9 @@
 //
 // Per pass: (1) block histograms of the digit, (2) one exclusive scan over
 // the digit-major [256][blocks] table, (3) a stable scatter in which each
-// wave ranks its 64 keys by matching digits with 9 ballots, and waves of a
-// block are ordered through LDS.  Stability across passes makes the result
+// wave ranks its 64 keys by matching digits with 9 ballots, waves of a block
+// are ordered through LDS, the block's tile is gathered in digit order in
+// LDS and then written out so that each digit's run is contiguous.  Stability across passes makes the result
 // independent of launch timing, so runs are bitwise reproducible.
 #include "sph_device.hpp"
 #include "sph_host.hpp"
@@ -73,40 +74,53 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   __shared__ uint32_t wcnt[4][256];
   __shared__ uint32_t woff[4][256];
   __shared__ uint32_t run[256];
-  __shared__ uint32_t gofs[256];
+  __shared__ uint32_t gofs[256];    // global position of this block's digit-d keys
+  __shared__ uint32_t lstart[256];  // their first position in the block's sorted tile
+  __shared__ uint32_t skey[kSortTile];
+  __shared__ uint32_t sval[kSortTile];
   const int tid = threadIdx.x;
   const int w = tid >> 6;
   const int lane = tid & 63;
+  const int b = blockIdx.x;
   run[tid] = 0;
-  // exclusive prefix of the digit totals, then this block's row offset
+  // exclusive prefix of the digit totals, this block's row offset, and the
+  // exclusive prefix of this block's own digit counts
   {
-    uint32_t t = tot[tid];
+    const uint32_t t = tot[tid];
+    const uint32_t o = offsets[(int64_t)tid * nblocks + b];
+    const uint32_t cnt = (b + 1 < nblocks ? offsets[(int64_t)tid * nblocks + b + 1] : t) - o;
     woff[0][tid] = t;
+    woff[1][tid] = cnt;
     __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {
-      uint32_t a = tid >= o ? woff[0][tid - o] : 0u;
+    for (int s = 1; s < 256; s <<= 1) {
+      const uint32_t a = tid >= s ? woff[0][tid - s] : 0u;
+      const uint32_t c = tid >= s ? woff[1][tid - s] : 0u;
       __syncthreads();
       woff[0][tid] += a;
+      woff[1][tid] += c;
       __syncthreads();
     }
-    gofs[tid] = woff[0][tid] - t + offsets[(int64_t)tid * nblocks + blockIdx.x];
+    gofs[tid] = woff[0][tid] - t + o;
+    lstart[tid] = woff[1][tid] - cnt;
     __syncthreads();
   }
-  const int64_t base = (int64_t)blockIdx.x * kSortTile;
+  const int64_t base = (int64_t)b * kSortTile;
+  const int count = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // stable ranking, 256 keys per round: the keys land in LDS in digit order
   for (int r = 0; r < kSortSub; ++r) {
-    const int64_t i = base + (int64_t)r * kSortThreads + tid;
-    const bool valid = i < n;
-    uint32_t key = valid ? kin[i] : 0u;
-    uint32_t val = valid ? (identity_vals ? (uint32_t)i : vin[i]) : 0u;
-    uint32_t d = valid ? ((key >> shift) & 255u) : 256u;
-    // lanes holding the same digit
+    const int li = r * kSortThreads + tid;
+    const bool valid = li < count;
+    const int64_t i = base + li;
+    const uint32_t key = valid ? kin[i] : 0u;
+    const uint32_t val = valid ? (identity_vals ? (uint32_t)i : vin[i]) : 0u;
+    const uint32_t d = valid ? ((key >> shift) & 255u) : 256u;
     unsigned long long peers = ~0ull;
 #pragma unroll
-    for (int b = 0; b < 9; ++b) {
-      const bool bit = (d >> b) & 1u;
-      const unsigned long long bal = ballot(bit);
-      peers &= bit ? bal : ~bal;
+    for (int bit = 0; bit < 9; ++bit) {
+      const bool on = (d >> bit) & 1u;
+      const unsigned long long bal = ballot(on);
+      peers &= on ? bal : ~bal;
     }
     const int rank = __popcll(peers & lt_mask);
     const bool leader = (peers & lt_mask) == 0ull;
@@ -116,8 +130,8 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     if (valid && leader) wcnt[w][d] = (uint32_t)__popcll(peers);
     __syncthreads();
     {
-      uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
-      uint32_t r0 = run[tid];
+      const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
+      const uint32_t r0 = run[tid];
       woff[0][tid] = r0;
       woff[1][tid] = r0 + c0;
       woff[2][tid] = r0 + c0 + c1;
@@ -126,11 +140,19 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     }
     __syncthreads();
     if (valid) {
-      const uint32_t dst = gofs[d] + woff[w][d] + (uint32_t)rank;
-      kout[dst] = key;
-      vout[dst] = val;
+      const uint32_t lp = lstart[d] + woff[w][d] + (uint32_t)rank;
+      skey[lp] = key;
+      sval[lp] = val;
     }
     __syncthreads();
+  }
+  // write out in tile order: runs of one digit go to consecutive addresses
+  for (int li = tid; li < count; li += kSortThreads) {
+    const uint32_t key = skey[li];
+    const uint32_t d = (key >> shift) & 255u;
+    const uint32_t dst = gofs[d] + ((uint32_t)li - lstart[d]);
+    kout[dst] = key;
+    vout[dst] = sval[li];
   }
 }
 
