@@ -176,7 +176,7 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
       (rc = h->alloc(&pb.partials, (int64_t)nitems * kNumRed)) ||
       (rc = h->alloc(&pb.stats, kStatSlots)) || (rc = h->alloc(&pb.err, 1)) ||
       (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
-      (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, np * kAnn)) ||
+      (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, (np + 63) / 64 * 64 * kAnn)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
       (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
       (rc = h->alloc(&pb.sort.counts, sort_counts_size(np))) || (rc = h->alloc(&pb.sort.tot, 256)) ||

@@ -659,7 +659,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const uint32_t tu = (uint32_t)t;
     const uint32_t selfpk = (tu << 4) | img_code(0, 0);
-    uint32_t* annrow = ann + t * kAnn;  // annulus candidates go straight out
+    // annulus candidates go straight out, entry e of query t at
+    // tile(t) * 64 kAnn + 64 e + t % 64, so the lanes' writes of one entry
+    // index fill the same lines
+    uint32_t* annrow = ann + (t >> 6) * (int64_t)(64 * kAnn) + (t & 63);
     // Lane groups: aligned blocks of consecutive queries.  A block whose
     // box holds more than kSplit candidates per query is halved (Morton
     // order can put two distant regions in one wave); blocks below 8 lanes
@@ -825,7 +828,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     cbuf.z = isc_ ? cbuf.w : cbuf.z;                                                    \
     cbuf.w = isc_ ? pkv_ : cbuf.w;                                                      \
     if (isc_ && (c_lo & 3) == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;              \
-    if (isa_ && c_an < kAnn) annrow[c_an] = pkv_;                                       \
+    if (isa_ && c_an < kAnn) annrow[64 * c_an] = pkv_;                                  \
     c_lo += isc_ ? 1 : 0;                                                               \
     c_an += isa_ ? 1 : 0;                                                               \
   }
@@ -957,11 +960,11 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const NbrRow row = nbr_row(nbr, t, kstride);
     // annulus squared distances, recomputed exactly as in the walk
-    const uint32_t* ai = ann + t * kAnn;
+    const uint32_t* ai = ann + (t >> 6) * (int64_t)(64 * kAnn) + (t & 63);  // stride 64
     uint32_t ap[kAnn];
     double ad[kAnn];
 #pragma unroll
-    for (int e = 0; e < kAnn; ++e) ap[e] = e < c_an ? ai[e] : (uint32_t)(t << 4) | 5u;
+    for (int e = 0; e < kAnn; ++e) ap[e] = e < c_an ? ai[64 * e] : (uint32_t)(t << 4) | 5u;
 #pragma unroll
     for (int e = 0; e < kAnn; ++e) {
       const uint32_t pk = ap[e];
