@@ -262,6 +262,7 @@ def run_serve(njobs, precision, capacity, max_jobs=256, load=0.7):
         "p50_ms": round(rs["p50_ms"], 2), "p99_ms": round(rs["p99_ms"], 2),
         "mean_latency_ms": round(float(np.mean(lat)), 2),
         "mean_occupancy": round(rs["occupancy"], 4), "ticks": int(rs["ticks"]),
+        "tick_ms": round(rs["tick_ms"], 3), "tick_gpu_ms": round(rs["tick_gpu_ms"], 3),
         "wall_s": round(rs["wall_s"], 2),
         "ics": "clustered recipe (16 Gaussians sigma 0.02 holding 80%, 20% uniform, "
                "v ~ N(0, 0.1^2), e = 1): per-job window of one of 16 shuffled 2^20 templates "

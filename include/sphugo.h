@@ -240,8 +240,9 @@ int sph_server_release(sph_server* s, int64_t job_id);
 /* jobs done, jobs failed, particle-steps done, seconds from the first submit
  * to the last completion, particle-steps/s over that span, p50 and p99
  * submit-to-done latency (ms), mean pool occupancy, ticks, jobs queued or
- * running now, mean tick duration (ms: uploads, one pass, readbacks). */
-#define SPH_SERVER_NSTATS 11
+ * running now, mean tick duration (ms: uploads, one pass, readbacks) and
+ * its device part (ms between events at the tick's start and end). */
+#define SPH_SERVER_NSTATS 12
 int sph_server_stats(sph_server* s, double out[SPH_SERVER_NSTATS]);
 
 const char* sph_last_error(void);

@@ -324,7 +324,7 @@ def autotune(sim):
 
 JOB_STATES = ["queued", "running", "done", "failed"]
 SERVER_STATS = ["jobs_done", "jobs_failed", "particle_steps", "wall_s", "particle_steps_per_s",
-                "p50_ms", "p99_ms", "occupancy", "ticks", "in_flight", "tick_ms"]
+                "p50_ms", "p99_ms", "occupancy", "ticks", "in_flight", "tick_ms", "tick_gpu_ms"]
 RESULT_FIELDS = ["x", "y", "vx", "vy", "rho", "e", "h"]
 
 

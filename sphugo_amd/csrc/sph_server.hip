@@ -115,6 +115,8 @@ struct sph_server {
   int64_t n_done = 0, n_failed = 0, particle_steps = 0, ticks = 0;
   double occupancy_sum = 0;
   double tick_ms_sum = 0;
+  double gpu_ms_sum = 0;
+  hipEvent_t ev_tick[2] = {nullptr, nullptr};
   bool have_t0 = false;
   Clock::time_point t0, t_last;
   std::vector<double> lat_ms;
@@ -306,6 +308,7 @@ bool tick(sph_server* s) {
   }
   const Clock::time_point t_tick = Clock::now();
   int rc = 0;
+  if (s->ev_tick[0]) hipEventRecord(s->ev_tick[0], c.st);
   for (int q : s->to_clear)
     if (!rc && !s->slots[q]) rc = clear_slot(s, q);
   s->to_clear.clear();
@@ -334,6 +337,7 @@ bool tick(sph_server* s) {
     hipError_t e = hipMemcpyAsync(&s->h_err, c.pb.err, sizeof(int), hipMemcpyDeviceToHost, c.st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s->h_stats, c.pb.stats, sizeof s->h_stats, hipMemcpyDeviceToHost, c.st);
+    if (e == hipSuccess && s->ev_tick[1]) e = hipEventRecord(s->ev_tick[1], c.st);
     if (e == hipSuccess) e = hipStreamSynchronize(c.st);
     if (e != hipSuccess) rc = fail(SPH_EDEVICE, "tick: %s", hipGetErrorString(e));
   }
@@ -354,6 +358,9 @@ bool tick(sph_server* s) {
   s->ticks += 1;
   s->occupancy_sum += (double)occupied / (double)s->cap;
   s->tick_ms_sum += ms_between(t_tick, Clock::now());
+  float gms = 0.f;
+  if (s->ev_tick[0] && hipEventElapsedTime(&gms, s->ev_tick[0], s->ev_tick[1]) == hipSuccess)
+    s->gpu_ms_sum += gms;
   const unsigned long long failures = sum_stats(s->h_stats).failures;
   const bool bad = s->h_err != 0 || failures != s->seen_failures;
   if (bad) {
@@ -481,6 +488,8 @@ extern "C" int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kma
   s->iota.resize(capacity);
   for (int64_t i = 0; i < capacity; ++i) s->iota[i] = (uint32_t)i;
   s->holes.push_back({0, capacity});
+  for (auto& e : s->ev_tick)
+    if (hipEventCreate(&e) != hipSuccess) return bail(fail(SPH_EDEVICE, "hipEventCreate failed"));
   c.bind();
   s->worker = std::thread(worker_main, s);
   *out = s;
@@ -496,6 +505,8 @@ extern "C" void sph_server_destroy(sph_server* s) {
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
   hipStreamSynchronize(s->core.st);
+  for (auto& e : s->ev_tick)
+    if (e) hipEventDestroy(e);
   for (auto& kv : s->jobs) delete kv.second;
   delete s;
 }
@@ -648,5 +659,6 @@ extern "C" int sph_server_stats(sph_server* s, double out[SPH_SERVER_NSTATS]) {
   out[8] = (double)s->ticks;
   out[9] = (double)(queued + running);
   out[10] = s->ticks ? s->tick_ms_sum / s->ticks : 0.0;
+  out[11] = s->ticks ? s->gpu_ms_sum / s->ticks : 0.0;
   return 0;
 }
