@@ -658,7 +658,6 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     u32x4 cbuf = {0u, 0u, 0u, 0u};
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const uint32_t tu = (uint32_t)t;
-    const uint32_t selfpk = (tu << 4) | img_code(0, 0);
     // annulus candidates go straight out, entry e of query t at
     // tile(t) * 64 kAnn + 64 e + t % 64, so the lanes' writes of one entry
     // index fill the same lines
@@ -814,56 +813,76 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         __builtin_amdgcn_wave_barrier();
         nm = (__builtin_amdgcn_readfirstlane(nm) + 3) & ~3;
         const uint8_t* mylist = ts.slist[my_sub];
-// Certain neighbours shift through a 4-entry register buffer (oldest in x)
-// that leaves as one 16-byte chunk store every fourth entry.
-#define SPH_ACCEPT(D2, PK, NOTSELF)                                                     \
+        // Blocks of 32 list positions: first every lane classifies its
+        // candidates into bit masks (certain: inside Rlo, annulus: up to
+        // Rhi), then emits the set bits in list order.  Self is dropped at
+        // emission (every image of the query itself).  Certain neighbours
+        // shift through a 4-entry register buffer (oldest in x) that leaves
+        // as one 16-byte chunk store every fourth entry.
+        for (int mb = 0; mb < nm; mb += 32) {
+          const int mend = nm < mb + 32 ? nm : mb + 32;
+          uint32_t cmask = 0u, amask = 0u;
+#define SPH_CLASSIFY(D2, BIT)                                                           \
   {                                                                                     \
     const double d2v_ = (D2);                                                           \
-    const uint32_t pkv_ = (PK);                                                         \
-    const bool ok_ = grp && d2v_ <= Rhi2 && (NOTSELF);                                  \
-    const bool isc_ = ok_ && d2v_ < Rlo2;                                               \
-    const bool isa_ = ok_ && !(d2v_ < Rlo2);                                            \
-    cbuf.x = isc_ ? cbuf.y : cbuf.x;                                                    \
-    cbuf.y = isc_ ? cbuf.z : cbuf.y;                                                    \
-    cbuf.z = isc_ ? cbuf.w : cbuf.z;                                                    \
-    cbuf.w = isc_ ? pkv_ : cbuf.w;                                                      \
-    if (isc_ && (c_lo & 3) == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;              \
-    if (isa_ && c_an < kAnn) annrow[64 * c_an] = pkv_;                                  \
-    c_lo += isc_ ? 1 : 0;                                                               \
-    c_an += isa_ ? 1 : 0;                                                               \
+    const bool in_ = grp && d2v_ <= Rhi2;                                               \
+    const bool lo_ = d2v_ < Rlo2;                                                       \
+    cmask |= (in_ && lo_) ? (BIT) : 0u;                                                 \
+    amask |= (in_ && !lo_) ? (BIT) : 0u;                                                \
   }
-        if (all_interior) {
-          for (int m = 0; m < nm; m += 4) {
-            const uint32_t i4 = *(const uint32_t*)(mylist + m);
-            const uint32_t i0 = i4 & 255u, i1 = (i4 >> 8) & 255u, i2 = (i4 >> 16) & 255u, i3 = i4 >> 24;
-            const double2 e0 = ts.sxy[i0], e1 = ts.sxy[i1], e2 = ts.sxy[i2], e3 = ts.sxy[i3];
-            const u32x4 p4 = {ts.spk[i0], ts.spk[i1], ts.spk[i2], ts.spk[i3]};
-            double a0 = xq - e0.x, b0 = yq - e0.y;
-            double a1 = xq - e1.x, b1 = yq - e1.y;
-            double a2 = xq - e2.x, b2 = yq - e2.y;
-            double a3 = xq - e3.x, b3 = yq - e3.y;
-            const double d0 = a0 * a0 + b0 * b0, d1 = a1 * a1 + b1 * b1;
-            const double d2_ = a2 * a2 + b2 * b2, d3 = a3 * a3 + b3 * b3;
-            // interior candidates: only the query itself at image 0 is self
-            SPH_ACCEPT(d0, p4.x, p4.x != selfpk);
-            SPH_ACCEPT(d1, p4.y, p4.y != selfpk);
-            SPH_ACCEPT(d2_, p4.z, p4.z != selfpk);
-            SPH_ACCEPT(d3, p4.w, p4.w != selfpk);
+          if (all_interior) {
+            for (int m = mb; m < mend; m += 4) {
+              const uint32_t i4 = *(const uint32_t*)(mylist + m);
+              const uint32_t i0 = i4 & 255u, i1 = (i4 >> 8) & 255u, i2 = (i4 >> 16) & 255u,
+                             i3 = i4 >> 24;
+              const double2 e0 = ts.sxy[i0], e1 = ts.sxy[i1], e2 = ts.sxy[i2], e3 = ts.sxy[i3];
+              const double a0 = xq - e0.x, b0 = yq - e0.y;
+              const double a1 = xq - e1.x, b1 = yq - e1.y;
+              const double a2 = xq - e2.x, b2 = yq - e2.y;
+              const double a3 = xq - e3.x, b3 = yq - e3.y;
+              const uint32_t bit = 1u << (m - mb);
+              SPH_CLASSIFY(a0 * a0 + b0 * b0, bit);
+              SPH_CLASSIFY(a1 * a1 + b1 * b1, bit << 1);
+              SPH_CLASSIFY(a2 * a2 + b2 * b2, bit << 2);
+              SPH_CLASSIFY(a3 * a3 + b3 * b3, bit << 3);
+            }
+          } else {
+            for (int m = mb; m < mend; ++m) {
+              const uint32_t im = mylist[m];
+              const uint32_t pk = ts.spk[im];
+              const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
+              const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
+              const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
+              const double2 e = ts.sxy[im];
+              const double ex = ax - e.x;
+              const double ey = ay - e.y;
+              SPH_CLASSIFY(ex * ex + ey * ey, 1u << (m - mb));
+            }
           }
-        } else {
-          for (int m = 0; m < nm; ++m) {
-            const uint32_t im = mylist[m];
-            const uint32_t pk = ts.spk[im];
-            const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
-            const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
-            const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
-            const double2 e = ts.sxy[im];
-            const double ex = ax - e.x;
-            const double ey = ay - e.y;
-            SPH_ACCEPT(ex * ex + ey * ey, pk, (pk >> 4) != tu);
+#undef SPH_CLASSIFY
+          while (cmask) {
+            const int bpos = __builtin_ctz(cmask);
+            cmask &= cmask - 1u;
+            const uint32_t pk = ts.spk[mylist[mb + bpos]];
+            if ((pk >> 4) != tu) {
+              cbuf.x = cbuf.y;
+              cbuf.y = cbuf.z;
+              cbuf.z = cbuf.w;
+              cbuf.w = pk;
+              if ((c_lo & 3) == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;
+              ++c_lo;
+            }
+          }
+          while (amask) {
+            const int bpos = __builtin_ctz(amask);
+            amask &= amask - 1u;
+            const uint32_t pk = ts.spk[mylist[mb + bpos]];
+            if ((pk >> 4) != tu) {
+              if (c_an < kAnn) annrow[64 * c_an] = pk;
+              ++c_an;
+            }
           }
         }
-#undef SPH_ACCEPT
         __builtin_amdgcn_wave_barrier();
       }
       pending &= ~gm;
