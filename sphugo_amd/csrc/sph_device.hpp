@@ -139,6 +139,9 @@ __device__ inline int64_t floordiv(int64_t a, int64_t b) {
   if ((a % b != 0) && ((a < 0) != (b < 0))) --q;
   return q;
 }
+// floor(a / 2^s): the arithmetic shift (every grid and level has 2^s cells
+// per axis, so no 64-bit division is needed on the device)
+__device__ inline int64_t floordiv_pow2(int64_t a, int s) { return a >> s; }
 
 // Reference kernel functions, evaluated in Go's order with no contraction.
 #pragma clang fp contract(off)
@@ -193,7 +196,7 @@ __device__ inline double rsq64(double x) {
 // ---- wave helpers (wave64) ----
 __device__ inline int lane_id() { return __lane_id(); }
 
-__device__ inline unsigned long long ballot(bool p) { return __ballot(p); }
+__device__ inline unsigned long long ballot(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 // number of set bits of m in lanes below this one
 __device__ inline int lane_prefix(unsigned long long m) {

@@ -80,8 +80,8 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
   }
   CellIdx c = cell_of(S, x, y);
   if (!c.ok) atomicOr(err, 1);
-  const int64_t cx = c.ux - floordiv(c.ux, S->gx) * S->gx;
-  const int64_t cy = c.uy - floordiv(c.uy, S->gy) * S->gy;
+  const int64_t cx = c.ux - (floordiv_pow2(c.ux, S->lg) << S->lg);
+  const int64_t cy = c.uy - (floordiv_pow2(c.uy, S->lg) << S->lg);
   keys[t] = (uint32_t)(S->coff + morton2((uint32_t)cx, (uint32_t)cy));
 }
 
@@ -155,7 +155,7 @@ __global__ __launch_bounds__(kItem) void k_permute(
   fout.h[t] = fin.h[i];
   fout.id[t] = fin.id[i];
   CellIdx c = cell_of(S, x, y);
-  const int64_t wx = floordiv(c.ux, S->gx), wy = floordiv(c.uy, S->gy);
+  const int64_t wx = floordiv_pow2(c.ux, S->lg), wy = floordiv_pow2(c.uy, S->lg);
   wrap[t] = (uint8_t)((wx + 2) | ((wy + 2) << 3));
 }
 
@@ -303,8 +303,8 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
         const double dx = xq < bx0 ? bx0 - xq : (xq > bx0 + scx ? xq - (bx0 + scx) : 0.0);
         const double dy = yq < by0 ? by0 - yq : (yq > by0 + scy ? yq - (by0 + scy) : 0.0);
         if (dx * dx + dy * dy <= Rm * Rm) {
-          tx = (int)floordiv(Xc, nc);
-          ty = (int)floordiv(Yc, nc);
+          tx = (int)floordiv_pow2(Xc, c);
+          ty = (int)floordiv_pow2(Yc, c);
           cell_range(S, cstart, c, (uint32_t)(Xc - tx * nc), (uint32_t)(Yc - ty * nc), a, b);
           have = b > a;
         }
@@ -472,7 +472,8 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
       if constexpr (!BITWISE) {
         row[pos] = pkr[s];
         if constexpr (FP32C) {
-          const float q = sqrtf((float)v[s]) * hinv_f;
+          // hardware square root (within 1 ulp), as in the walk's finish
+          const float q = __builtin_amdgcn_sqrtf((float)v[s]) * hinv_f;
           const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
           const float y = f - fc;
           const float tt = fs + y;
@@ -587,8 +588,8 @@ __device__ inline void write_result(const SimDev* S, int64_t t, double xq, doubl
 __device__ inline double local_radius(const SimDev* S, const uint32_t* __restrict__ cstart,
                                       double xq, double yq) {
   const CellIdx ci = cell_of(S, xq, yq);
-  const int64_t fx = ci.ux - floordiv(ci.ux, S->gx) * S->gx;
-  const int64_t fy = ci.uy - floordiv(ci.uy, S->gy) * S->gy;
+  const int64_t fx = ci.ux - (floordiv_pow2(ci.ux, S->lg) << S->lg);
+  const int64_t fy = ci.uy - (floordiv_pow2(ci.uy, S->lg) << S->lg);
   int c = S->lg - 1 > 0 ? S->lg - 1 : 0;
   for (;; --c) {
     const int sh = S->lg - c;
@@ -700,8 +701,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         int tx = 0, ty = 0;
         if (ci < ncell) {
           const int64_t Xc = X0 + ci % nX, Yc = Y0 + ci / nX;
-          tx = (int)floordiv(Xc, nc);
-          ty = (int)floordiv(Yc, nc);
+          tx = (int)floordiv_pow2(Xc, c);
+          ty = (int)floordiv_pow2(Yc, c);
           cell_range(S, cstart, c, (uint32_t)(Xc - tx * nc), (uint32_t)(Yc - ty * nc), pa, pb);
           have = pb > pa;
         }
@@ -814,21 +815,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         nm = (__builtin_amdgcn_readfirstlane(nm) + 3) & ~3;
         const uint8_t* mylist = ts.slist[my_sub];
         // Blocks of 32 list positions: first every lane classifies its
-        // candidates into bit masks (certain: inside Rlo, annulus: up to
-        // Rhi), then emits the set bits in list order.  Self is dropped at
-        // emission (every image of the query itself).  Certain neighbours
-        // shift through a 4-entry register buffer (oldest in x) that leaves
-        // as one 16-byte chunk store every fourth entry.
+        // candidates into bit masks, shifting one bit per position into
+        // each (certain: inside Rlo, reach: up to Rhi; x + x + carry is one
+        // add-with-carry per bit), then emits the set bits in list order
+        // (position m of the block sits at bit mend - 1 - m).  Self is
+        // dropped at emission (every image of the query itself).  Certain
+        // neighbours shift through a 4-entry register buffer (oldest in x)
+        // that leaves as one 16-byte chunk store every fourth entry.
         for (int mb = 0; mb < nm; mb += 32) {
           const int mend = nm < mb + 32 ? nm : mb + 32;
-          uint32_t cmask = 0u, amask = 0u;
-#define SPH_CLASSIFY(D2, BIT)                                                           \
+          uint32_t cmask = 0u, hmask = 0u;
+#define SPH_CLASSIFY(D2)                                                                \
   {                                                                                     \
     const double d2v_ = (D2);                                                           \
-    const bool in_ = grp && d2v_ <= Rhi2;                                               \
-    const bool lo_ = d2v_ < Rlo2;                                                       \
-    cmask |= (in_ && lo_) ? (BIT) : 0u;                                                 \
-    amask |= (in_ && !lo_) ? (BIT) : 0u;                                                \
+    cmask = cmask + cmask + (d2v_ < Rlo2 ? 1u : 0u);                                    \
+    hmask = hmask + hmask + (d2v_ <= Rhi2 ? 1u : 0u);                                   \
   }
           if (all_interior) {
             for (int m = mb; m < mend; m += 4) {
@@ -840,11 +841,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
               const double a1 = xq - e1.x, b1 = yq - e1.y;
               const double a2 = xq - e2.x, b2 = yq - e2.y;
               const double a3 = xq - e3.x, b3 = yq - e3.y;
-              const uint32_t bit = 1u << (m - mb);
-              SPH_CLASSIFY(a0 * a0 + b0 * b0, bit);
-              SPH_CLASSIFY(a1 * a1 + b1 * b1, bit << 1);
-              SPH_CLASSIFY(a2 * a2 + b2 * b2, bit << 2);
-              SPH_CLASSIFY(a3 * a3 + b3 * b3, bit << 3);
+              SPH_CLASSIFY(a0 * a0 + b0 * b0);
+              SPH_CLASSIFY(a1 * a1 + b1 * b1);
+              SPH_CLASSIFY(a2 * a2 + b2 * b2);
+              SPH_CLASSIFY(a3 * a3 + b3 * b3);
             }
           } else {
             for (int m = mb; m < mend; ++m) {
@@ -856,14 +856,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
               const double2 e = ts.sxy[im];
               const double ex = ax - e.x;
               const double ey = ay - e.y;
-              SPH_CLASSIFY(ex * ex + ey * ey, 1u << (m - mb));
+              SPH_CLASSIFY(ex * ex + ey * ey);
             }
           }
 #undef SPH_CLASSIFY
+          if (!grp) cmask = hmask = 0u;
+          uint32_t amask = hmask & ~cmask;
           while (cmask) {
-            const int bpos = __builtin_ctz(cmask);
-            cmask &= cmask - 1u;
-            const uint32_t pk = ts.spk[mylist[mb + bpos]];
+            const int bpos = 31 - __builtin_clz(cmask);
+            cmask ^= 1u << bpos;
+            const uint32_t pk = ts.spk[mylist[mend - 1 - bpos]];
             if ((pk >> 4) != tu) {
               cbuf.x = cbuf.y;
               cbuf.y = cbuf.z;
@@ -874,9 +876,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             }
           }
           while (amask) {
-            const int bpos = __builtin_ctz(amask);
-            amask &= amask - 1u;
-            const uint32_t pk = ts.spk[mylist[mb + bpos]];
+            const int bpos = 31 - __builtin_clz(amask);
+            amask ^= 1u << bpos;
+            const uint32_t pk = ts.spk[mylist[mend - 1 - bpos]];
             if ((pk >> 4) != tu) {
               if (c_an < kAnn) annrow[64 * c_an] = pk;
               ++c_an;
@@ -1067,7 +1069,9 @@ __global__ __launch_bounds__(256) void k_knn_finish(
         const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
         const double ex = (xq + ox) - pj[u].x, ey = (yq + oy) - pj[u].y;
         if constexpr (FP32C) {
-          const float q = sqrtf((float)(ex * ex + ey * ey)) * hinv;
+          // hardware square root (within 1 ulp): the FP32 form's error budget
+          // holds it, and it saves the correctly rounded sequence per pair
+          const float q = __builtin_amdgcn_sqrtf((float)(ex * ex + ey * ey)) * hinv;
           const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
           const float y = f - c_f;
           const float tt = s_f + y;
