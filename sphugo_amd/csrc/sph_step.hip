@@ -216,6 +216,8 @@ struct TileShared {
   uint32_t spk[65];
   double sbb[kSub][4];       // box of each sub-tile (x0, x1, y0, y1)
   uint8_t slist[kSub][64];   // staged candidates within reach of each sub-tile
+  unsigned long long smask[kSub];  // staged slots in each sub-tile's list
+  unsigned long long selfm[64];    // per query: staged slots that are itself
 };
 
 union WaveShared {
@@ -657,8 +659,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     int c_lo = 0, c_an = 0;
     // certain neighbours collect in cbuf and leave in 16-byte stores
     u32x4 cbuf = {0u, 0u, 0u, 0u};
+    u32x4* cptr = row.chunk(0);  // next chunk of the row to fill
     const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
-    const uint32_t tu = (uint32_t)t;
     // annulus candidates go straight out, entry e of query t at
     // tile(t) * 64 kAnn + 64 e + t % 64, so the lanes' writes of one entry
     // index fill the same lines
@@ -770,6 +772,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       }
       int nsteps = 0;
       while ((1 << nsteps) < npieces) ++nsteps;
+      ts.selfm[lane] = 0ull;
+      const int64_t wave_t0 = it.first + (int64_t)wv * 64;
       for (int base = 0; base < ncand; base += 64) {
         // stage 64 candidates
         const int u = base + lane;
@@ -791,6 +795,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             interior = (ix == 0) && (iy == 0);
           }
           cpk = (j << 4) | img_code(ix, iy);
+          // an image of one of this wave's own queries: mark it in that
+          // query's self mask (ordered after the query's own clear below)
+          const int64_t so = (int64_t)j - wave_t0;
+          if (so >= 0 && so < 64) atomicOr(&ts.selfm[so], 1ull << lane);
         }
         ts.sxy[lane] = make_double2(cxv, cyv);
         ts.spk[lane] = cpk;
@@ -807,6 +815,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             const bool in = px >= sbx0[g] && px <= sbx1[g] && py >= sby0[g] && py <= sby1[g];
             const unsigned long long m = ballot(in);
             if (in) ts.slist[g][lane_prefix(m)] = (uint8_t)lane;
+            if (lane == 0) ts.smask[g] = m;
             const int c = __popcll(m);
             nm = c > nm ? c : nm;
           }
@@ -814,14 +823,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         __builtin_amdgcn_wave_barrier();
         nm = (__builtin_amdgcn_readfirstlane(nm) + 3) & ~3;
         const uint8_t* mylist = ts.slist[my_sub];
+        // list positions of this query's own images (every image of the
+        // query itself is excluded): a staged slot u in the sub-tile's list
+        // sits at the count of list slots below u
+        unsigned long long spos = 0ull;
+        {
+          const unsigned long long msub = ts.smask[my_sub];
+          unsigned long long sm = ts.selfm[lane] & msub;
+          ts.selfm[lane] = 0ull;
+          while (sm) {
+            const int us = __builtin_ctzll(sm);
+            sm &= sm - 1ull;
+            spos |= 1ull << __popcll(msub & ((1ull << us) - 1ull));
+          }
+        }
         // Blocks of 32 list positions: first every lane classifies its
         // candidates into bit masks, shifting one bit per position into
-        // each (certain: inside Rlo, reach: up to Rhi; x + x + carry is one
-        // add-with-carry per bit), then emits the set bits in list order
-        // (position m of the block sits at bit mend - 1 - m).  Self is
-        // dropped at emission (every image of the query itself).  Certain
-        // neighbours shift through a 4-entry register buffer (oldest in x)
-        // that leaves as one 16-byte chunk store every fourth entry.
+        // each (certain: inside Rlo, reach: up to Rhi), then emits the set
+        // bits in list order.  Certain neighbours shift through a 4-entry
+        // register buffer (oldest in x) that leaves as one 16-byte chunk
+        // store every fourth entry.
         for (int mb = 0; mb < nm; mb += 32) {
           const int mend = nm < mb + 32 ? nm : mb + 32;
           uint32_t cmask = 0u, hmask = 0u;
@@ -860,29 +881,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             }
           }
 #undef SPH_CLASSIFY
-          if (!grp) cmask = hmask = 0u;
-          uint32_t amask = hmask & ~cmask;
+          // position m of the block was shifted in at bit mend - 1 - m;
+          // reversed and shifted down, it sits at bit m - mb
+          const int sh = 32 - (mend - mb);
+          const uint32_t keep = grp ? ~(uint32_t)(spos >> mb) : 0u;
+          cmask = (__builtin_bitreverse32(cmask) >> sh) & keep;
+          uint32_t amask = (__builtin_bitreverse32(hmask) >> sh) & keep & ~cmask;
+          const uint8_t* blist = mylist + mb;
           while (cmask) {
-            const int bpos = 31 - __builtin_clz(cmask);
-            cmask ^= 1u << bpos;
-            const uint32_t pk = ts.spk[mylist[mend - 1 - bpos]];
-            if ((pk >> 4) != tu) {
-              cbuf.x = cbuf.y;
-              cbuf.y = cbuf.z;
-              cbuf.z = cbuf.w;
-              cbuf.w = pk;
-              if ((c_lo & 3) == 3 && c_lo < k) *row.chunk(c_lo >> 2) = cbuf;
-              ++c_lo;
+            const uint32_t pk = ts.spk[blist[__builtin_ctz(cmask)]];
+            cmask &= cmask - 1u;
+            cbuf.x = cbuf.y;
+            cbuf.y = cbuf.z;
+            cbuf.z = cbuf.w;
+            cbuf.w = pk;
+            if ((c_lo & 3) == 3 && c_lo < k) {
+              *cptr = cbuf;
+              cptr += 64;
             }
+            ++c_lo;
           }
           while (amask) {
-            const int bpos = 31 - __builtin_clz(amask);
-            amask ^= 1u << bpos;
-            const uint32_t pk = ts.spk[mylist[mend - 1 - bpos]];
-            if ((pk >> 4) != tu) {
-              if (c_an < kAnn) annrow[64 * c_an] = pk;
-              ++c_an;
-            }
+            const uint32_t pk = ts.spk[blist[__builtin_ctz(amask)]];
+            amask &= amask - 1u;
+            if (c_an < kAnn) annrow[64 * c_an] = pk;
+            ++c_an;
           }
         }
         __builtin_amdgcn_wave_barrier();
