@@ -1293,7 +1293,11 @@ __global__ __launch_bounds__(256) void k_force(
           ry = (float)((pj.y - oy) - yi);
         }
         const float d2 = rx * rx + ry * ry;
-        const float dinv = __builtin_amdgcn_rsqf(d2);
+        // Two particles closer than the fixed-point step (L / 2^32) have a
+        // zero displacement here; their pair term vanishes with the distance
+        // (DF(q) ~ q and dot ~ r as r -> 0), so it is taken as 0 rather than
+        // as the 0 * inf of the reciprocal square root.
+        const float dinv = d2 > 0.f ? __builtin_amdgcn_rsqf(d2) : 0.f;
         float q = (d2 * dinv) * hinv;
         q = q > 1.f ? 1.f : q;
         const float dR = kern_DFf(kern, q);

@@ -1,7 +1,8 @@
 """Edge cases of the HIP path against the oracle (GPU): gravity and
 reflection walls, the Wendland kernel over several steps, anisotropic and
-shifted boxes, exact distance ties on lattices, the smallest k and N, and a
-batch whose simulations have different boxes."""
+shifted boxes, exact distance ties on lattices, the smallest k and N, a
+batch whose simulations have different boxes, and a pair of particles closer
+than the FP32C force record's fixed-point step."""
 import numpy as np
 import pytest
 
@@ -115,6 +116,30 @@ def test_smallest_k_and_n():
         g, o = _pair(st, k)
         _check_knn(g, o)
         _check_forces(g, o)
+
+
+def test_fp32c_pair_closer_than_fixed_point_step():
+    # Two approaching particles inside one L / 2^32 step of the force
+    # record's fixed-point positions: their displacement there is zero, and
+    # the pair term (which vanishes with the distance) must not become NaN.
+    st = dict(ics.uniform(2200, seed=5))
+    for f in ("x", "y", "vx", "vy"):
+        st[f] = st[f].copy()
+    x0 = np.floor(0.3 * 2.0**32) / 2.0**32 + 2.0**-34
+    y0 = np.floor(0.6 * 2.0**32) / 2.0**32 + 2.0**-34
+    st["x"][0], st["y"][0] = x0, y0
+    st["x"][1], st["y"][1] = x0 + 2.0**-35, y0
+    st["vx"][0], st["vx"][1] = 0.1, -0.1
+    g, o = _pair(st, 32, precision=L.PREC_FP32C)
+    _check_knn(g, o)
+    for f in ("rho", "ax", "ay", "edot"):
+        a, b = g.get(f), o.get(f)
+        assert np.isfinite(a).all(), f
+        assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max(), f
+    # and the state stays finite over steps
+    g.step(3)
+    for f in ("x", "y", "vx", "vy", "e"):
+        assert np.isfinite(g.get(f)).all(), f
 
 
 def test_batch_with_different_boxes():
