@@ -172,6 +172,21 @@ int sph_knn_stats(sph_sim* s, int64_t* retries, int64_t* ties);
  * annulus over capacity), tiles sent to the wavefront search. */
 int sph_knn_counters(sph_sim* s, int64_t out[7]);
 
+/* ---- checkpoint and resume (no reference counterpart: the reference keeps
+ * no state on disk; its Animator holds rendered frames only,
+ * sim/animator.go:111-130) ----
+ * A checkpoint is one byte image: a 64-byte header ("SPHCKPT1", version, N,
+ * step, ...), the creation parameters, the simulation's device scalars
+ * (half step, CFL minimum, totals) and the 14 per-particle FP64 arrays
+ * (x y vx vy e ax ay edot vpx vpy epred h rho c) in the device's slot order
+ * followed by the particle IDs (u32).  A handle loaded from it continues
+ * bit for bit as the saved one would have.  One simulation per handle
+ * (SPH_EINVAL for a batch); the caller owns the buffer.  Load validates the
+ * header and sizes and returns SPH_EINVAL for anything else. */
+int sph_checkpoint_size(sph_sim* s, int64_t* bytes);
+int sph_checkpoint_save(sph_sim* s, void* buf, int64_t bytes);
+int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out);
+
 /* ---- batches of independent simulations (no reference counterpart) ----
  * One handle holding `count` simulations; every kernel of a step covers all
  * of them in one launch over (simulation, tile) work items.  Precision and

@@ -88,6 +88,9 @@ SIGNATURES = {
     "sph_batch_read_field": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, _D]),
     "sph_batch_write_field": (ctypes.c_int, [_VP, ctypes.c_int32, ctypes.c_int32, _D]),
     "sph_batch_reductions": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
+    "sph_checkpoint_size": (ctypes.c_int, [_VP, _I64]),
+    "sph_checkpoint_save": (ctypes.c_int, [_VP, ctypes.c_void_p, ctypes.c_int64]),
+    "sph_checkpoint_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(_VP)]),
     "sph_set_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "sph_autotune": (ctypes.c_int, [_VP, _D, _D]),
     "sph_server_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
@@ -179,6 +182,35 @@ class Sim:
         if getattr(self, "_h", None):
             lib().sph_destroy(self._h)
             self._h = None
+
+    # checkpoint / resume (sph_checkpoint_*)
+    def checkpoint(self) -> bytes:
+        size = ctypes.c_int64()
+        check(lib().sph_checkpoint_size(self._h, ctypes.byref(size)))
+        buf = ctypes.create_string_buffer(size.value)
+        check(lib().sph_checkpoint_save(self._h, buf, size.value))
+        return buf.raw
+
+    def save(self, path):
+        with open(path, "wb") as f:
+            f.write(self.checkpoint())
+
+    @classmethod
+    def from_checkpoint(cls, data: bytes) -> "Sim":
+        data = data if isinstance(data, bytes) else bytes(data)
+        h = _VP()
+        check(lib().sph_checkpoint_load(data, len(data), ctypes.byref(h)))
+        self = cls.__new__(cls)
+        self._h = h
+        self.params = Params.from_buffer_copy(bytes(data[64:64 + ctypes.sizeof(Params)]))
+        self.k = self.params.k
+        self.n = int(lib().sph_num_particles(h))
+        return self
+
+    @classmethod
+    def load(cls, path) -> "Sim":
+        with open(path, "rb") as f:
+            return cls.from_checkpoint(f.read())
 
     def __del__(self):
         self.close()

@@ -114,6 +114,17 @@ class Simulation {
         dev_(o.dev_) {
     o.dev_ = nullptr;
   }
+  Simulation& operator=(Simulation&& o) noexcept {
+    if (this != &o) {
+      sph_destroy(dev_);
+      Config = o.Config;
+      Particles = std::move(o.Particles);
+      CurrentStep = o.CurrentStep;
+      dev_ = o.dev_;
+      o.dev_ = nullptr;
+    }
+    return *this;
+  }
   ~Simulation() { sph_destroy(dev_); }
 
   // sim/sph.go:56-61
@@ -162,6 +173,29 @@ class Simulation {
   }
 
   sph_sim* Device() { return handle(); }
+
+  // Checkpoint of the device state (sph_checkpoint_save): a byte image from
+  // which Resume continues bit for bit.  The reference keeps no state on
+  // disk; this has no counterpart there.
+  std::vector<unsigned char> Checkpoint() {
+    int64_t bytes = 0;
+    check(sph_checkpoint_size(handle(), &bytes));
+    std::vector<unsigned char> buf((size_t)bytes);
+    check(sph_checkpoint_save(dev_, buf.data(), bytes));
+    return buf;
+  }
+
+  // A simulation continued from a checkpoint: the device parameters come
+  // from the image; conf supplies the host-side fields (NSteps).
+  static Simulation Resume(const SphConfig& conf, const std::vector<unsigned char>& buf) {
+    sph_sim* h = nullptr;
+    check(sph_checkpoint_load(buf.data(), (int64_t)buf.size(), &h));
+    Simulation s(conf, std::vector<Particle>((size_t)sph_num_particles(h)));
+    s.dev_ = h;
+    s.CurrentStep = (int)sph_current_step(h);
+    s.pull();
+    return s;
+  }
 
  private:
   sph_sim* dev_ = nullptr;

@@ -1,7 +1,10 @@
 // Runs the small uniform case through the C++ mirror of package `sim`
 // (sim.hpp) and prints the reference's totals as one JSON line.
 //
-//   sim_demo [n=2200] [steps=10] [seed=12345678] [k=32]
+//   sim_demo [n=2200] [steps=10] [seed=12345678] [k=32] [resume_at=-1]
+//
+// With resume_at >= 0 the run is checkpointed after resume_at steps and
+// continued from the checkpoint in a new Simulation (same output expected).
 //
 // Initial conditions: SplitMix64 counter stream (value i = mix(seed +
 // (i+1) * 0x9E3779B97F4A7C15)), top 53 bits scaled by 2^-53; x takes values
@@ -25,6 +28,7 @@ int main(int argc, char** argv) {
   const int steps = argc > 2 ? std::atoi(argv[2]) : 10;
   const uint64_t seed = argc > 3 ? std::strtoull(argv[3], nullptr, 10) : 12345678ull;
   const int k = argc > 4 ? std::atoi(argv[4]) : 32;
+  const int resume_at = argc > 5 ? std::atoi(argv[5]) : -1;
   std::vector<sim::Particle> ps((size_t)n);
   for (int64_t i = 0; i < n; ++i) {
     ps[i].Pos.X = (double)(splitmix(seed, (uint64_t)i + 1) >> 11) * 0x1p-53;
@@ -45,7 +49,14 @@ int main(int argc, char** argv) {
   conf.ViscBeta = 2.0;
   try {
     sim::Simulation s = sim::MakeSimulationFromConf(conf, std::move(ps));
-    s.Run();
+    if (resume_at >= 0 && resume_at <= steps) {
+      for (int i = 0; i < resume_at; ++i) s.Step();
+      const std::vector<unsigned char> ckpt = s.Checkpoint();
+      s = sim::Simulation::Resume(conf, ckpt);
+      for (int i = resume_at; i < steps; ++i) s.Step();
+    } else {
+      s.Run();
+    }
     double sx = 0;
     for (const sim::Particle& q : s.Particles) sx += q.Pos.X;
     std::printf("{\"n\": %lld, \"steps\": %d, \"step\": %d, \"TotalEnergy\": %.17g, "

@@ -232,6 +232,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
     fill_simdev(d, ps[s], ns[s], lg);
     d.off = off;
     h->hsims.push_back(d);
+    h->hparams.push_back(*ps[s]);
     h->sim_n.push_back(ns[s]);
     red_first[s] = (int)h->hitems.size();
     for (int64_t b = 0; b < ns[s]; b += kItem) {
@@ -688,6 +689,160 @@ extern "C" int sph_batch_reductions(sph_sim* s, int32_t sim, double out[6]) {
   const SimDev& d = s->hsims[sim];
   for (int r = 0; r < kNumRed; ++r) out[r] = d.red[r];
   out[5] = d.dt_last;
+  return 0;
+}
+
+// ------------------------------------------------------------------------
+// Checkpoint / resume: header, creation parameters, the simulation's device
+// scalars, then every per-particle array in slot order and the IDs.  Slot
+// order (not ID order) is what the stable sort of the next pass starts from,
+// so a loaded handle continues bit for bit.
+
+namespace {
+constexpr char kCkptMagic[8] = {'S', 'P', 'H', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+constexpr int kCkptFields = 14;  // x y vx vy e ax ay edot vpx vpy ep h rho c
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t header_bytes;
+  int64_t n;
+  int64_t step;
+  int32_t have_h;
+  uint32_t params_bytes;
+  uint32_t simdev_bytes;
+  uint32_t nfields;
+  double grid_cpp;
+  uint64_t reserved;
+};
+static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
+
+int64_t ckpt_bytes(int64_t n) {
+  return (int64_t)(sizeof(CkptHeader) + sizeof(sph_params) + sizeof(SimDev)) +
+         n * (int64_t)(kCkptFields * sizeof(double) + sizeof(uint32_t));
+}
+
+// device arrays of the payload, in payload order (the state between steps:
+// positions in P[0], everything else in the current field set)
+void ckpt_arrays(sph_sim* s, double* a[kCkptFields]) {
+  const Fields& f = s->F[s->cur];
+  double* v[kCkptFields] = {s->P[0].x, s->P[0].y, f.vx,  f.vy, f.e, f.ax,  f.ay,
+                            f.edot,    f.vpx,     f.vpy, f.ep, f.h, f.rho, f.c};
+  for (int i = 0; i < kCkptFields; ++i) a[i] = v[i];
+}
+}  // namespace
+
+extern "C" int sph_checkpoint_size(sph_sim* s, int64_t* bytes) {
+  if (!s || !bytes) return fail(SPH_EINVAL, "null argument");
+  if (s->hsims.size() != 1) return fail(SPH_EINVAL, "a checkpoint holds one simulation, not a batch");
+  *bytes = ckpt_bytes(s->np);
+  return 0;
+}
+
+extern "C" int sph_checkpoint_save(sph_sim* s, void* buf, int64_t bytes) {
+  if (!s || !buf) return fail(SPH_EINVAL, "null argument");
+  if (s->hsims.size() != 1) return fail(SPH_EINVAL, "a checkpoint holds one simulation, not a batch");
+  const int64_t n = s->np;
+  if (bytes < ckpt_bytes(n))
+    return fail(SPH_EINVAL, "checkpoint buffer of %lld bytes, %lld needed", (long long)bytes,
+                (long long)ckpt_bytes(n));
+  int rc = fetch_sims(s);  // also waits for the handle's work
+  if (rc) return rc;
+  CkptHeader hd;
+  std::memset(&hd, 0, sizeof hd);
+  std::memcpy(hd.magic, kCkptMagic, sizeof hd.magic);
+  hd.version = kCkptVersion;
+  hd.header_bytes = sizeof hd;
+  hd.n = n;
+  hd.step = s->step;
+  hd.have_h = s->pb.have_h;
+  hd.params_bytes = sizeof(sph_params);
+  hd.simdev_bytes = sizeof(SimDev);
+  hd.nfields = kCkptFields;
+  hd.grid_cpp = s->grid_cpp;
+  char* p = (char*)buf;
+  std::memcpy(p, &hd, sizeof hd);
+  p += sizeof hd;
+  std::memcpy(p, &s->hparams[0], sizeof(sph_params));
+  p += sizeof(sph_params);
+  std::memcpy(p, &s->hsims[0], sizeof(SimDev));
+  p += sizeof(SimDev);
+  double* a[kCkptFields];
+  ckpt_arrays(s, a);
+  for (int i = 0; i < kCkptFields; ++i) {
+    HIPCHK(hipMemcpyAsync(p, a[i], n * sizeof(double), hipMemcpyDeviceToHost, s->st));
+    p += n * sizeof(double);
+  }
+  HIPCHK(hipMemcpyAsync(p, s->F[s->cur].id, n * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out) {
+  if (!buf || !out) return fail(SPH_EINVAL, "null argument");
+  *out = nullptr;
+  CkptHeader hd;
+  if (bytes < (int64_t)sizeof hd) return fail(SPH_EINVAL, "not a checkpoint: %lld bytes", (long long)bytes);
+  std::memcpy(&hd, buf, sizeof hd);
+  if (std::memcmp(hd.magic, kCkptMagic, sizeof hd.magic) != 0)
+    return fail(SPH_EINVAL, "not a checkpoint (bad magic)");
+  if (hd.version != kCkptVersion || hd.header_bytes != sizeof hd ||
+      hd.params_bytes != sizeof(sph_params) || hd.simdev_bytes != sizeof(SimDev) ||
+      hd.nfields != kCkptFields)
+    return fail(SPH_EINVAL, "checkpoint version %u or layout is not this build's", hd.version);
+  if (hd.n < 1 || hd.n >= (1LL << 28) || hd.step < 0 || (hd.have_h != 0 && hd.have_h != 1))
+    return fail(SPH_EINVAL, "corrupt checkpoint header");
+  const int64_t n = hd.n;
+  if (bytes < ckpt_bytes(n))
+    return fail(SPH_EINVAL, "truncated checkpoint: %lld bytes, %lld needed", (long long)bytes,
+                (long long)ckpt_bytes(n));
+  bool known_grid = false;
+  for (double cpp : kTuneCpp) known_grid = known_grid || cpp == hd.grid_cpp;
+  if (!known_grid) return fail(SPH_EINVAL, "corrupt checkpoint header (grid)");
+  const char* p = (const char*)buf + sizeof hd;
+  sph_params prm;
+  std::memcpy(&prm, p, sizeof prm);
+  p += sizeof prm;
+  SimDev sd;
+  std::memcpy(&sd, p, sizeof sd);
+  p += sizeof sd;
+  if (sd.n != n || sd.off != 0 || sd.k != prm.k)
+    return fail(SPH_EINVAL, "corrupt checkpoint (simulation record)");
+  // aligned host copies of what sph_create takes; the rest goes straight up
+  std::vector<double> init(5 * n);
+  for (int i = 0; i < 5; ++i) std::memcpy(init.data() + i * n, p + i * n * sizeof(double), n * sizeof(double));
+  std::vector<uint32_t> ids(n);
+  std::memcpy(ids.data(), p + kCkptFields * n * sizeof(double), n * sizeof(uint32_t));
+  const double* xs[5];
+  for (int i = 0; i < 5; ++i) xs[i] = init.data() + i * n;
+  const uint32_t* idp = ids.data();
+  const sph_params* pp = &prm;
+  sph_sim* h = nullptr;
+  int rc = create_batch(1, &pp, &n, &xs[0], &xs[1], &xs[2], &xs[3], &xs[4], &idp, &h);
+  if (rc) return rc;
+  auto bail = [&](int code) {
+    sph_destroy(h);
+    return code;
+  };
+  layout_cells(h, hd.grid_cpp);
+  if (h->hsims[0].lg != sd.lg || h->hsims[0].coff != sd.coff)
+    return bail(fail(SPH_EINVAL, "corrupt checkpoint (grid of the simulation record)"));
+  h->hsims[0] = sd;
+  double* a[kCkptFields];
+  ckpt_arrays(h, a);
+  for (int i = 5; i < kCkptFields; ++i)
+    if (hipMemcpyAsync(a[i], p + i * n * sizeof(double), n * sizeof(double), hipMemcpyHostToDevice,
+                       h->st) != hipSuccess)
+      return bail(fail(SPH_EDEVICE, "upload failed"));
+  if (hipMemcpyAsync(h->pb.sims, h->hsims.data(), sizeof(SimDev), hipMemcpyHostToDevice, h->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(h->st) != hipSuccess)
+    return bail(fail(SPH_EDEVICE, "upload failed"));
+  h->step = hd.step;
+  h->pb.have_h = hd.have_h;
+  h->bind();
+  *out = h;
   return 0;
 }
 

@@ -34,6 +34,7 @@ using sph::SimDev;
 struct sph_sim {
   int device = 0;
   std::vector<SimDev> hsims;
+  std::vector<sph_params> hparams;  // creation parameters, per simulation
   std::vector<int64_t> sim_n;
   std::vector<Item> hitems;
   int64_t np = 0;

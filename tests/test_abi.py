@@ -77,3 +77,28 @@ def test_null_handles():
     assert lib.sph_server_wait(None, 1, 0.0) == -1
     lib.sph_destroy(None)
     lib.sph_server_destroy(None)
+
+
+def test_checkpoint_load_rejects_bad_images():
+    # validated before any device call
+    import struct
+    lib = L.lib()
+    h = ctypes.c_void_p()
+
+    def load(data):
+        rc = lib.sph_checkpoint_load(data, len(data), ctypes.byref(h))
+        return rc, lib.sph_last_error().decode()
+
+    assert load(b"SPHCKPT1")[0] == -1  # shorter than the header
+    rc, msg = load(b"NOTACKPT" + bytes(120))
+    assert rc == -1 and "magic" in msg
+    hdr = struct.pack("<8sIIqqiIIIdQ", b"SPHCKPT1", 2, 64, 1000, 0, 0,
+                      ctypes.sizeof(L.Params), 0, 14, 4.0, 0)
+    rc, msg = load(hdr + bytes(64))
+    assert rc == -1 and "version" in msg
+    hdr = struct.pack("<8sIIqqiIIIdQ", b"SPHCKPT1", 1, 64, -5, 0, 0,
+                      ctypes.sizeof(L.Params), 0, 14, 4.0, 0)
+    assert load(hdr + bytes(64))[0] == -1
+    assert lib.sph_checkpoint_size(None, None) == -1
+    assert lib.sph_checkpoint_save(None, None, 0) == -1
+    assert h.value is None

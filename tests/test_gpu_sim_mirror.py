@@ -39,3 +39,15 @@ def test_sim_mirror_matches_fixture():
     assert out["sum_x"] == x_ref
     assert out["TotalMomentum"] == float(np.sqrt(vx * vx + vy * vy))
     assert out["h_first"] == float(gold["cfl10_h"][0])
+
+
+def test_sim_mirror_resume_from_checkpoint():
+    # Simulation.Checkpoint after 4 steps, Simulation::Resume, 6 more steps:
+    # the same totals, bit for bit, as the uninterrupted run
+    outs = []
+    for extra in ([], ["4"]):
+        r = subprocess.run([DEMO, "2200", "10", "12345678", "32"] + extra, capture_output=True,
+                           text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        outs.append(json.loads(r.stdout))
+    assert outs[0] == outs[1]
