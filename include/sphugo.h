@@ -121,6 +121,16 @@ int sph_create(const sph_params* p, int64_t n, const double* x, const double* y,
                const uint32_t* id, sph_sim** out);
 void sph_destroy(sph_sim* s);
 
+/* Sources (sim/sph.go:71-86, the spawners of sim/config-parser.go:58-102
+ * running on the host): append m particles with the next IDs n..n+m-1.
+ * vx, vy, e may be NULL (zero); every other field of a new particle starts
+ * at zero, as Go's zero value, and its first neighbour search is the exact
+ * one.  Call between steps (the reference spawns at the start of Step); the
+ * half step already set for the next step is kept.  One simulation per
+ * handle (SPH_EINVAL for a batch). */
+int sph_append(sph_sim* s, int64_t m, const double* x, const double* y,
+               const double* vx, const double* vy, const double* e);
+
 /* nsteps calls of Simulation.Step (sim/sph.go:64-198), including the extra
  * force pass of step 0; returns when the device has finished. */
 int sph_step(sph_sim* s, int32_t nsteps);

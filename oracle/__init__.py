@@ -92,6 +92,7 @@ def lib():
         "orc_tree_free": (None, [vp]),
         "orc_create": (vp, [ctypes.POINTER(Params), ctypes.c_int64, _D, _D, _D, _D, _D]),
         "orc_destroy": (None, [vp]),
+        "orc_append": (ctypes.c_int32, [vp, ctypes.c_int64, _D, _D, _D, _D, _D]),
         "orc_set_threads": (None, [vp, ctypes.c_int32]),
         "orc_step": (ctypes.c_int32, [vp, ctypes.c_int32]),
         "orc_calculate_forces": (ctypes.c_int32, [vp]),
@@ -210,6 +211,14 @@ class OracleSim:
     def _check(self, rc):
         if rc != 0:
             raise RuntimeError(lib().orc_last_error().decode())
+
+    def append(self, x, y, vx=None, vy=None, e=None):
+        """Particles of a source, appended with the next IDs (sim/sph.go:71-86)."""
+        m = len(x)
+        arrs = [_c64(a) if a is not None else None for a in (x, y, vx, vy, e)]
+        self._check(lib().orc_append(self._h, m, *[_dp(a) if a is not None else None
+                                                     for a in arrs]))
+        self.n += m
 
     def step(self, nsteps: int = 1):
         self._check(lib().orc_step(self._h, nsteps))

@@ -362,6 +362,58 @@ orc_sim* orc_create(const orc_params* p, int64_t n, const double* x,
   return s;
 }
 
+/* Sources, sim/sph.go:71-86: the particles a spawner produced are appended
+ * to Root.Particles (next IDs; velocity and energy as given, every other
+ * field zero, as the Go zero value), and the root is rebuilt by MakeCells
+ * (sim/core.go:93-105): a fresh cell over [0,1]^2 without children. */
+int32_t orc_append(orc_sim* s, int64_t m, const double* x, const double* y,
+                   const double* vx, const double* vy, const double* e) {
+  if (m < 0 || (m > 0 && (!x || !y))) { set_err("orc_append: bad arguments"); return -1; }
+  if (m == 0) return 0;
+  const int64_t n = s->n, n2 = n + m;
+  const size_t k = (size_t)s->k, nk = (size_t)n * k, nk2 = (size_t)n2 * k;
+  OP* ps = (OP*)realloc(s->ps, (size_t)n2 * sizeof(OP));
+  double* nnd = (double*)realloc(s->nnd, nk2 * sizeof(double));
+  int64_t* nni = (int64_t*)realloc(s->nni, nk2 * sizeof(int64_t));
+  double* nnpx = (double*)realloc(s->nnpx, nk2 * sizeof(double));
+  double* nnpy = (double*)realloc(s->nnpy, nk2 * sizeof(double));
+  int32_t* nnimg = (int32_t*)realloc(s->nnimg, nk2 * sizeof(int32_t));
+  double* nnd2 = (double*)realloc(s->nnd2, nk2 * sizeof(double));
+  int64_t* poi = (int64_t*)realloc(s->pos_of_id, (size_t)n2 * sizeof(int64_t));
+  if (ps) s->ps = ps;
+  if (nnd) s->nnd = nnd;
+  if (nni) s->nni = nni;
+  if (nnpx) s->nnpx = nnpx;
+  if (nnpy) s->nnpy = nnpy;
+  if (nnimg) s->nnimg = nnimg;
+  if (nnd2) s->nnd2 = nnd2;
+  if (poi) s->pos_of_id = poi;
+  if (!ps || !nnd || !nni || !nnpx || !nnpy || !nnimg || !nnd2 || !poi) {
+    set_err("orc_append: out of memory");
+    return -1;
+  }
+  memset(&s->ps[n], 0, (size_t)m * sizeof(OP));
+  for (int64_t i = 0; i < m; i++) {
+    OP* q = &s->ps[n + i];
+    q->x = x[i]; q->y = y[i];
+    q->vx = vx ? vx[i] : 0; q->vy = vy ? vy[i] : 0;
+    q->e = e ? e[i] : 0;
+    q->id = n + i;
+  }
+  memset(s->nnd + nk, 0, (nk2 - nk) * sizeof(double));
+  for (size_t i = nk; i < nk2; i++) s->nni[i] = -1;
+  memset(s->nnpx + nk, 0, (nk2 - nk) * sizeof(double));
+  memset(s->nnpy + nk, 0, (nk2 - nk) * sizeof(double));
+  memset(s->nnimg + nk, 0, (nk2 - nk) * sizeof(int32_t));
+  memset(s->nnd2 + nk, 0, (nk2 - nk) * sizeof(double));
+  s->n = n2;
+  s->cv.n = 1;
+  OCell* r = &s->cv.c[s->root];
+  r->begin = 0; r->end = n2; r->llx = 0; r->lly = 0; r->urx = 1; r->ury = 1;
+  r->lower = r->upper = -1;
+  return 0;
+}
+
 void orc_destroy(orc_sim* s) {
   if (!s) return;
   free(s->ps); free(s->nnd); free(s->nni); free(s->nnpx); free(s->nnpy);
@@ -773,6 +825,7 @@ double orc_cfl_dt_half(const orc_sim* s) {
   for (int64_t i = 0; i < s->n; i++) {
     const OP* p = &s->ps[i];
     double h = s->nnd[p->id * s->k];
+    if (!(h > 0)) continue; /* appended since the last pass: no neighbours yet */
     double v = sqrt(p->vx * p->vx + p->vy * p->vy);
     double t = (0.5 * h) / (p->c + v);
     if (t < m) m = t;
@@ -780,7 +833,8 @@ double orc_cfl_dt_half(const orc_sim* s) {
   return 0.5 * (s->p.cfl * m);
 }
 
-/* Step, sim/sph.go:64-198 (sources are not part of this restatement) */
+/* Step, sim/sph.go:64-198 (a source's particles come in by orc_append
+ * before the step, sim/sph.go:71-86) */
 int32_t orc_step(orc_sim* s, int32_t nsteps) {
   int64_t n = s->n;
   for (int32_t st = 0; st < nsteps; st++) {

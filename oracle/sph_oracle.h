@@ -89,6 +89,10 @@ orc_sim* orc_create(const orc_params* p, int64_t n, const double* x,
                     const double* y, const double* vx, const double* vy,
                     const double* e);
 void orc_destroy(orc_sim* s);
+/* Sources (sim/sph.go:71-86): append m particles with the next IDs (vx, vy,
+ * e may be NULL).  0 on success. */
+int32_t orc_append(orc_sim* s, int64_t m, const double* x, const double* y,
+                   const double* vx, const double* vy, const double* e);
 void orc_set_threads(orc_sim* s, int32_t nthreads);
 /* Simulation.Step() nsteps times (sim/sph.go:64-198). 0 on success. */
 int32_t orc_step(orc_sim* s, int32_t nsteps);

@@ -60,6 +60,7 @@ SIGNATURES = {
     "sph_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.c_int64, _D, _D, _D, _D, _D,
                                   _U32, ctypes.POINTER(_VP)]),
     "sph_destroy": (None, [_VP]),
+    "sph_append": (ctypes.c_int, [_VP, ctypes.c_int64, _D, _D, _D, _D, _D]),
     "sph_step": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "sph_step_async": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "sph_synchronize": (ctypes.c_int, [_VP]),
@@ -182,6 +183,14 @@ class Sim:
         if getattr(self, "_h", None):
             lib().sph_destroy(self._h)
             self._h = None
+
+    def append(self, x, y, vx=None, vy=None, e=None):
+        """A source's particles, appended with the next IDs (sph_append)."""
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float64)
+                for a in (x, y, vx, vy, e)]
+        check(lib().sph_append(self._h, len(arrs[0]),
+                               *[None if a is None else dptr(a) for a in arrs]))
+        self.n = int(lib().sph_num_particles(self._h))
 
     # checkpoint / resume (sph_checkpoint_*)
     def checkpoint(self) -> bytes:

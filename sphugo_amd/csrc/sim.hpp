@@ -174,6 +174,25 @@ class Simulation {
 
   sph_sim* Device() { return handle(); }
 
+  // The particles a source spawned (sim/sph.go:71-86: appended to
+  // Root.Particles before the step); their IDs continue the slice's.
+  void Append(const std::vector<Particle>& ps) {
+    if (ps.empty()) return;
+    const size_t m = ps.size();
+    std::vector<double> buf(5 * m);
+    for (size_t i = 0; i < m; ++i) {
+      buf[i] = ps[i].Pos.X;
+      buf[m + i] = ps[i].Pos.Y;
+      buf[2 * m + i] = ps[i].Vel.X;
+      buf[3 * m + i] = ps[i].Vel.Y;
+      buf[4 * m + i] = ps[i].E;
+    }
+    check(sph_append(handle(), (int64_t)m, buf.data(), buf.data() + m, buf.data() + 2 * m,
+                     buf.data() + 3 * m, buf.data() + 4 * m));
+    Particles.insert(Particles.end(), ps.begin(), ps.end());
+    pull();
+  }
+
   // Checkpoint of the device state (sph_checkpoint_save): a byte image from
   // which Resume continues bit for bit.  The reference keeps no state on
   // disk; this has no counterpart there.

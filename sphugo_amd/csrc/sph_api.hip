@@ -846,5 +846,86 @@ extern "C" int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out
   return 0;
 }
 
+// Sources (sim/sph.go:71-86): m particles appended with the next IDs.  The
+// handle is rebuilt around the grown state (slot order kept, new particles
+// after it) and swapped into the caller's handle.
+extern "C" int sph_append(sph_sim* s, int64_t m, const double* x, const double* y,
+                          const double* vx, const double* vy, const double* e) {
+  if (!s) return fail(SPH_EINVAL, "null handle");
+  if (m < 0 || (m > 0 && (!x || !y))) return fail(SPH_EINVAL, "append: bad arguments");
+  if (m == 0) return 0;
+  if (s->hsims.size() != 1)
+    return fail(SPH_EINVAL, "particles are appended to one simulation, not a batch");
+  const int64_t n = s->np, n2 = n + m;
+  if (n2 >= (1LL << 28)) return fail(SPH_EINVAL, "too many particles in one simulation");
+  for (int64_t i = 0; i < m; ++i)
+    if (!std::isfinite(x[i]) || !std::isfinite(y[i]))
+      return fail(SPH_ERANGE, "non-finite position of appended particle %lld", (long long)i);
+  int rc = fetch_sims(s);
+  if (rc) return rc;
+  std::vector<double> a((size_t)kCkptFields * n2, 0.0);
+  std::vector<uint32_t> ids(n2);
+  double* src[kCkptFields];
+  ckpt_arrays(s, src);
+  for (int f = 0; f < kCkptFields; ++f)
+    HIPCHK(hipMemcpyAsync(a.data() + f * n2, src[f], n * sizeof(double), hipMemcpyDeviceToHost,
+                          s->st));
+  HIPCHK(hipMemcpyAsync(ids.data(), s->F[s->cur].id, n * sizeof(uint32_t), hipMemcpyDeviceToHost,
+                        s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  for (int64_t i = 0; i < m; ++i) {
+    a[0 * n2 + n + i] = x[i];
+    a[1 * n2 + n + i] = y[i];
+    a[2 * n2 + n + i] = vx ? vx[i] : 0.0;
+    a[3 * n2 + n + i] = vy ? vy[i] : 0.0;
+    a[4 * n2 + n + i] = e ? e[i] : 0.0;
+    ids[n + i] = (uint32_t)(n + i);
+  }
+  const sph_params prm = s->hparams[0];
+  const sph_params* pp = &prm;
+  const double* xs[5];
+  for (int i = 0; i < 5; ++i) xs[i] = a.data() + i * n2;
+  const uint32_t* idp = ids.data();
+  sph_sim* h = nullptr;
+  if ((rc = create_batch(1, &pp, &n2, &xs[0], &xs[1], &xs[2], &xs[3], &xs[4], &idp, &h)))
+    return rc;
+  auto bail = [&](int code) {
+    sph_destroy(h);
+    return code;
+  };
+  layout_cells(h, s->grid_cpp);  // same cells per particle as before
+  // device-updated scalars carry over (the next half step was set by the
+  // last pass, over the particles that had neighbours)
+  SimDev& d = h->hsims[0];
+  const SimDev& o = s->hsims[0];
+  d.dt_half = o.dt_half;
+  d.dt_last = o.dt_last;
+  d.cfl_min_bits = o.cfl_min_bits;
+  for (int r = 0; r < kNumRed; ++r) d.red[r] = o.red[r];
+  d.last_speed = o.last_speed;
+  double* dst[kCkptFields];
+  ckpt_arrays(h, dst);
+  for (int f = 5; f < kCkptFields; ++f)
+    if (hipMemcpyAsync(dst[f], a.data() + f * n2, n2 * sizeof(double), hipMemcpyHostToDevice,
+                       h->st) != hipSuccess)
+      return bail(fail(SPH_EDEVICE, "upload failed"));
+  if (hipMemcpyAsync(h->pb.sims, h->hsims.data(), sizeof(SimDev), hipMemcpyHostToDevice, h->st) !=
+          hipSuccess ||
+      hipStreamSynchronize(h->st) != hipSuccess)
+    return bail(fail(SPH_EDEVICE, "upload failed"));
+  if (s->timing && (rc = sph_set_timing(h, 1))) return bail(rc);
+  h->step = s->step;
+  h->pb.have_h = s->pb.have_h;
+  h->graph_on = s->graph_on;
+  for (int t = 0; t < SPH_NTIMERS; ++t) {
+    h->t_ms[t] = s->t_ms[t];
+    h->t_cnt[t] = s->t_cnt[t];
+  }
+  h->bind();
+  s->swap_state(*h);
+  sph_destroy(h);  // the old buffers
+  return 0;
+}
+
 extern "C" const char* sph_last_error(void) { return g_err.c_str(); }
 extern "C" const char* sph_version(void) { return "sphugo_amd 0.1 (gfx950)"; }

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "sph_host.hpp"
@@ -90,6 +91,38 @@ struct sph_sim {
     pb.fcur = F[cur];
     pb.fnew = F[cur ^ 1];
     pb.pos_cur = P[0];
+  }
+  // Exchange everything with another handle (a handle rebuilt in place:
+  // the caller's pointer stays valid).  Lists every member above.
+  void swap_state(sph_sim& o) {
+    std::swap(device, o.device);
+    hsims.swap(o.hsims);
+    hparams.swap(o.hparams);
+    sim_n.swap(o.sim_n);
+    hitems.swap(o.hitems);
+    std::swap(np, o.np);
+    std::swap(kmax, o.kmax);
+    std::swap(prec, o.prec);
+    std::swap(bitwise, o.bitwise);
+    std::swap(timing, o.timing);
+    std::swap(st, o.st);
+    std::swap(ev, o.ev);
+    std::swap(t_ms, o.t_ms);
+    std::swap(t_cnt, o.t_cnt);
+    std::swap(cur, o.cur);
+    std::swap(F, o.F);
+    std::swap(P, o.P);
+    std::swap(pb, o.pb);
+    std::swap(step, o.step);
+    allocs.swap(o.allocs);
+    std::swap(d_scratch, o.d_scratch);
+    std::swap(d_nids, o.d_nids);
+    std::swap(d_nd2, o.d_nd2);
+    std::swap(last_stats, o.last_stats);
+    std::swap(graph_on, o.graph_on);
+    std::swap(gexec, o.gexec);
+    std::swap(grid_cpp, o.grid_cpp);
+    std::swap(cell_capacity, o.cell_capacity);
   }
 };
 
