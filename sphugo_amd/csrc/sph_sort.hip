@@ -1,9 +1,10 @@
 // Stable LSD radix sort of (cell key, particle index) pairs, 8-bit digits.
 //
 // Replaces the reference's per-pass Treebuild/Partition permutation
-// (sim/core.go:126-224): particles are reordered by the row-major cell index
-// of a uniform grid so that every row segment of cells is one contiguous
-// range of the state arrays.
+// (sim/core.go:126-224): particles are reordered by the Morton (Z-order)
+// index of their cell in a uniform 2^lg x 2^lg grid, so that every aligned
+// square block of cells, at every level, is one contiguous range of the
+// state arrays.
 //
 // Per pass: (1) block histograms of the digit, (2) one exclusive scan over
 // the digit-major [256][blocks] table, (3) a stable scatter in which each
