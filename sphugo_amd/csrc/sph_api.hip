@@ -174,7 +174,7 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
       (rc = h->alloc((char**)&pb.rec,
                      np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) ||
       (rc = h->alloc(&pb.partials, (int64_t)nitems * kNumRed)) ||
-      (rc = h->alloc(&pb.stats, kStatSlots)) || (rc = h->alloc(&pb.err, 1)) ||
+      (rc = h->alloc(&pb.stats, kStatSlots)) || (rc = h->alloc(&pb.err, 1 + pb.nsims)) ||
       (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
       (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, (np + 63) / 64 * 64 * kAnn)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
@@ -293,7 +293,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
         return bail(fail(SPH_EDEVICE, "memset failed"));
   }
   if (hipMemsetAsync(pb.stats, 0, kStatSlots * sizeof(KnnStats), h->st) != hipSuccess ||
-      hipMemsetAsync(pb.err, 0, sizeof(int), h->st) != hipSuccess ||
+      hipMemsetAsync(pb.err, 0, (1 + pb.nsims) * sizeof(int), h->st) != hipSuccess ||
       hipMemsetAsync(pb.nbr, 0, nbr_entries(np, pb.kmax) * sizeof(uint32_t), h->st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "memset failed"));
   std::vector<uint32_t> idbuf;

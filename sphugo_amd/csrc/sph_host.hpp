@@ -66,7 +66,8 @@ struct PassBuffers {
   uint32_t* knn_cnt;       // per query: certain | annulus << 8, or ~0 (retry)
   uint32_t* knn_ann;       // per query: annulus candidates (kAnn entries)
   int have_h;             // every particle has an h from an earlier pass
-  int* err;             // device error flag
+  int* err;             // device error flags: [0] any, [1 + s] simulation s (1: a
+                        // particle left the images, 2: kNN found fewer than k)
   SortBuffers sort;
 };
 

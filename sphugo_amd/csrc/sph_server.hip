@@ -102,7 +102,7 @@ struct sph_server {
   int* d_rf = nullptr;
   int* d_rc = nullptr;
   double* d_stage = nullptr;  // kNumRes rows of capacity
-  int h_err = 0;
+  std::vector<int> h_err;  // error flags: [0] any, [1 + slot] per job slot
   KnnStats h_stats[kStatSlots] = {};
   unsigned long long seen_failures = 0;
   // slots freed and item lists changed at the end of a tick, uploaded at the
@@ -267,15 +267,16 @@ int enqueue_readback(sph_server* s, Job* j) {
   return 0;
 }
 
+void fail_job(sph_server* s, Job* j, const std::string& msg) {
+  j->state = SPH_JOB_FAILED;
+  j->error = msg;
+  j->t_done = Clock::now();
+  s->n_failed += 1;
+}
+
 void fail_all_running(sph_server* s, const std::string& msg) {
-  for (int q = 0; q < s->max_jobs; ++q) {
-    Job* j = s->slots[q];
-    if (!j) continue;
-    j->state = SPH_JOB_FAILED;
-    j->error = msg;
-    j->t_done = Clock::now();
-    s->n_failed += 1;
-  }
+  for (int q = 0; q < s->max_jobs; ++q)
+    if (s->slots[q]) fail_job(s, s->slots[q], msg);
 }
 
 // One tick: admission, one pass over the pool, completions.
@@ -334,7 +335,8 @@ bool tick(sph_server* s) {
   for (Job* j : finishing)
     if (!rc) rc = enqueue_readback(s, j);
   if (!rc) {
-    hipError_t e = hipMemcpyAsync(&s->h_err, c.pb.err, sizeof(int), hipMemcpyDeviceToHost, c.st);
+    hipError_t e = hipMemcpyAsync(s->h_err.data(), c.pb.err, s->h_err.size() * sizeof(int),
+                                  hipMemcpyDeviceToHost, c.st);
     if (e == hipSuccess)
       e = hipMemcpyAsync(s->h_stats, c.pb.stats, sizeof s->h_stats, hipMemcpyDeviceToHost, c.st);
     if (e == hipSuccess && s->ev_tick[1]) e = hipEventRecord(s->ev_tick[1], c.st);
@@ -362,15 +364,26 @@ bool tick(sph_server* s) {
   if (s->ev_tick[0] && hipEventElapsedTime(&gms, s->ev_tick[0], s->ev_tick[1]) == hipSuccess)
     s->gpu_ms_sum += gms;
   const unsigned long long failures = sum_stats(s->h_stats).failures;
-  const bool bad = s->h_err != 0 || failures != s->seen_failures;
-  if (bad) {
-    // the flags are pool-wide: every job of this pass is failed
-    fail_all_running(s, s->h_err ? "a particle left the periodic images [-L, 2L) of the box"
-                                 : "kNN found fewer than k neighbours");
+  bool flagged = s->h_err[0] != 0;
+  for (int q = 0; q < s->max_jobs && !flagged; ++q) flagged = s->h_err[1 + q] != 0;
+  if (flagged || failures != s->seen_failures) {
+    // each flag names the job slot whose particle or search failed: only
+    // those jobs fail, the others of the pass go on
+    bool owned = false;
+    for (int q = 0; q < s->max_jobs; ++q) {
+      const int f = s->h_err[1 + q];
+      if (!f || !s->slots[q]) continue;
+      fail_job(s, s->slots[q], (f & 1) ? "a particle left the periodic images [-L, 2L) of the box"
+                                       : "kNN found fewer than k neighbours");
+      owned = true;
+    }
+    if (!owned)
+      fail_all_running(s, s->h_err[0] ? "a particle left the periodic images [-L, 2L) of the box"
+                                      : "kNN found fewer than k neighbours");
     s->seen_failures = failures;
-    if (s->h_err && hipMemsetAsync(c.pb.err, 0, sizeof(int), c.st) != hipSuccess) {
+    if (flagged && hipMemsetAsync(c.pb.err, 0, s->h_err.size() * sizeof(int), c.st) != hipSuccess) {
       s->fatal = SPH_EDEVICE;
-      s->fatal_msg = "could not clear the error flag";
+      s->fatal_msg = "could not clear the error flags";
     }
   }
   const Clock::time_point now = Clock::now();
@@ -476,11 +489,12 @@ extern "C" int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kma
   pb.nitems_all = 0;
   if (hipMemsetAsync(d_sims, 0, max_jobs * sizeof(SimDev), c.st) != hipSuccess ||
       hipMemsetAsync(pb.stats, 0, kStatSlots * sizeof(KnnStats), c.st) != hipSuccess ||
-      hipMemsetAsync(pb.err, 0, sizeof(int), c.st) != hipSuccess ||
+      hipMemsetAsync(pb.err, 0, (1 + max_jobs) * sizeof(int), c.st) != hipSuccess ||
       hipMemsetAsync(pb.nbr, 0, nbr_entries(capacity, pb.kmax) * sizeof(uint32_t), c.st) != hipSuccess ||
       hipStreamSynchronize(c.st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "server init failed"));
   s->slots.assign(max_jobs, nullptr);
+  s->h_err.assign(1 + max_jobs, 0);
   s->hsim.assign(max_jobs, SimDev{});
   s->hsim_rb.assign(max_jobs, SimDev{});
   s->h_rf.assign(max_jobs, 0);

@@ -79,7 +79,10 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
     y = y + VY[t] * dt;
   }
   CellIdx c = cell_of(S, x, y);
-  if (!c.ok) atomicOr(err, 1);
+  if (!c.ok) {
+    atomicOr(err, 1);
+    atomicOr(err + 1 + it.sim, 1);
+  }
   const int64_t cx = c.ux - (floordiv_pow2(c.ux, S->lg) << S->lg);
   const int64_t cy = c.uy - (floordiv_pow2(c.uy, S->lg) << S->lg);
   keys[t] = (uint32_t)(S->coff + morton2((uint32_t)cx, (uint32_t)cy));
@@ -1128,7 +1131,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const double* __restrict__ VPY, const uint32_t* __restrict__ ID,
     uint32_t* __restrict__ nbr, int kstride, double* __restrict__ RHO,
     double* __restrict__ C, void* __restrict__ recp, KnnStats* __restrict__ stats,
-    const uint2* __restrict__ retry, const uint32_t* __restrict__ retry_count, int all_mode) {
+    const uint2* __restrict__ retry, const uint32_t* __restrict__ retry_count, int all_mode,
+    int* __restrict__ err) {
   __shared__ SearchShared wsh[4];
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
@@ -1153,8 +1157,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     const double2 pq = XY[t];
     const double xq = pq.x, yq = pq.y, hp = H[t];
     const double R0 = hp > 0 ? hp * 1.1 : local_radius(S, cstart, xq, yq);
+    const unsigned long long nfail0 = nfail;
     const SearchOut so = wave_search<BITWISE, FP32C>(S, t, xq, yq, R0, XY, WR, cstart, ID,
                                                      nbr_row(nbr, t, kstride), sh, nretry, nties, nfail);
+    if (lane == 0 && nfail != nfail0) atomicOr(err + 1 + items[item].sim, 2);
     if (lane == 0 && so.tau > 0)
       write_result<FP32C>(S, t, xq, yq, so.tau, so.facc, EP, VPX, VPY, H, RHO, C, recp);
   }
@@ -1490,7 +1496,7 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   hipLaunchKernelGGL((k_knn_fallback<BW, F32>), dim3(fb_grid), dim3(256), 0, st, b.items,     \
                      b.nitems, b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h,                       \
                      b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr, b.kmax, b.fnew.rho,   \
-                     b.fnew.c, b.rec, b.stats, b.retry, b.retry_count, fastpath ? 0 : 1)
+                     b.fnew.c, b.rec, b.stats, b.retry, b.retry_count, fastpath ? 0 : 1, b.err)
     mark(6, 0);
     if (fastpath) {
       if (b.prec == kPrecFP32C) SPH_KNN(true); else SPH_KNN(false);

@@ -126,3 +126,32 @@ def test_server_rejects():
     with pytest.raises(L.SphError):
         srv.info(jid)
     srv.close()
+
+
+def test_server_failure_stays_with_its_job():
+    # A job whose particle leaves the periodic images fails alone (its slot's
+    # flag names it); the jobs sharing its passes finish with the results
+    # they have alone.
+    srv = L.Server(capacity=12000, max_jobs=4, kmax=32)
+    good = []
+    for i in range(3):
+        st = _state(i, 2500, "blast")
+        p = _p(st)
+        good.append((srv.submit(p, st, 6), p, st))
+    bad_st = dict(_state(9, 2000, "blast"))
+    bad_st["vx"] = bad_st["vx"].copy()
+    bad_st["vx"][17] = 1.0e5  # with a fixed half step of 1e-3 it leaves at once
+    bad = srv.submit(_p(bad_st, dt_half=1e-3), bad_st, 6)
+    srv.drain()
+    info = srv.info(bad)
+    assert info["state"] == "failed", info
+    with pytest.raises(L.SphError, match="left the periodic images"):
+        srv.wait(bad)
+    for jid, p, st in good:
+        assert srv.info(jid)["state"] == "done"
+        ref, red = _alone(p, st, 6)
+        for f in L.RESULT_FIELDS:
+            assert (srv.result(jid, f) == ref[f]).all(), f
+    s = srv.stats()
+    assert s["jobs_done"] == 3 and s["jobs_failed"] == 1
+    srv.close()
