@@ -849,11 +849,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         for (int mb = 0; mb < nm; mb += 32) {
           const int mend = nm < mb + 32 ? nm : mb + 32;
           uint32_t cmask = 0u, hmask = 0u;
+// One position: compare into VCC, then add-with-carry mask + mask + VCC
+// shifts the bit in (one instruction per mask where the compiler emits a
+// select and a shift-or).  s_nop 1 gives the two wait states an FP64
+// compare's VCC write needs before a VALU reads it.
 #define SPH_CLASSIFY(D2)                                                                \
   {                                                                                     \
     const double d2v_ = (D2);                                                           \
-    cmask = cmask + cmask + (d2v_ < Rlo2 ? 1u : 0u);                                    \
-    hmask = hmask + hmask + (d2v_ <= Rhi2 ? 1u : 0u);                                   \
+    asm("v_cmp_lt_f64 vcc, %1, %2\n\ts_nop 1\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc"         \
+        : "+v"(cmask) : "v"(d2v_), "v"(Rlo2) : "vcc");                                  \
+    asm("v_cmp_le_f64 vcc, %1, %2\n\ts_nop 1\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc"         \
+        : "+v"(hmask) : "v"(d2v_), "v"(Rhi2) : "vcc");                                  \
   }
           if (all_interior) {
             for (int m = mb; m < mend; m += 4) {
