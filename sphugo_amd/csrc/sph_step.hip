@@ -777,9 +777,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       while ((1 << nsteps) < npieces) ++nsteps;
       ts.selfm[lane] = 0ull;
       const int64_t wave_t0 = it.first + (int64_t)wv * 64;
+      // Stage s takes every nst-th candidate (lane l: candidate l * nst + s),
+      // so each stage samples the whole box evenly and the eight sub-tile
+      // lists come out about equally long: every lane runs the longest one.
+      const int nst = (ncand + 63) >> 6;
       for (int base = 0; base < ncand; base += 64) {
         // stage 64 candidates
-        const int u = base + lane;
+        const int u = lane * nst + (base >> 6);
         double cxv = NAN, cyv = 0.0;
         uint32_t cpk = 0;
         bool interior = true;
