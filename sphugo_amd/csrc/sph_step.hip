@@ -191,7 +191,7 @@ __device__ inline void cell_range(const SimDev* S, const uint32_t* __restrict__ 
 
 constexpr int kV1Cells = 128;  // coarse cells of one search (<= 9 x 9 used)
 constexpr int kTileCells = 256;  // coarse cells of one tile box (<= 15 x 15 used)
-constexpr double kDelta = 0.03;  // bracket half-width around the last h
+constexpr double kDelta = 0.025;  // bracket half-width around the last h
 constexpr int kSplit = 24;       // max tile candidates per query before a split
 constexpr int kSub = 8;          // sub-tiles of a wave (8 queries each)
 constexpr int kSubW = 64 / kSub;
