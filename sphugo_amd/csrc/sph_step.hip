@@ -1032,33 +1032,29 @@ __global__ __launch_bounds__(256) void k_knn_finish(
       const double ex = ax - pj.x, ey = ay - pj.y;
       ad[e] = e < c_an ? ex * ex + ey * ey : INFINITY;
     }
-    // r smallest by (d2, slot)
-    double pd = -1.0;
-    int ps = -1;
-    for (int s_ = 0; s_ < r; ++s_) {
-      double bd = INFINITY;
-      int bs = kAnn;
-      uint32_t bpk = 0;
+    // the r smallest by (d2, slot): entry e is taken when fewer than r
+    // entries precede it in that order, and goes to row position c_lo plus
+    // that count (a fixed 16 x 15 comparisons, where a selection loop runs
+    // as often as the largest r of the wave)
+    double tau = -1.0;
+    uint32_t taken = 0u;
 #pragma unroll
-      for (int e = 0; e < kAnn; ++e) {
-        const double d = ad[e];
-        const bool after = d > pd || (d == pd && e > ps);
-        const bool before = d < bd;
-        if (after && before) {
-          bd = d;
-          bs = e;
-          bpk = ap[e];
-        }
+    for (int e = 0; e < kAnn; ++e) {
+      int before = 0;
+#pragma unroll
+      for (int f = 0; f < kAnn; ++f)
+        if (f != e) before += (f < e ? ad[f] <= ad[e] : ad[f] < ad[e]) ? 1 : 0;
+      if (e < c_an && before < r) {
+        row[c_lo + before] = ap[e];
+        tau = ad[e] > tau ? ad[e] : tau;
+        taken |= 1u << e;
       }
-      row[c_lo + s_] = bpk;
-      pd = bd;
-      ps = bs;
     }
-    double tau = pd;
     // an annulus entry left out with d2 == tau: resolve the tie exactly
     bool tie = false;
 #pragma unroll
-    for (int e = 0; e < kAnn; ++e) tie = tie || (e > ps && ad[e] == tau);
+    for (int e = 0; e < kAnn; ++e)
+      tie = tie || (e < c_an && !((taken >> e) & 1u) && ad[e] == tau);
     if (tie) {
       ++nties;
       int below = 0;
