@@ -781,6 +781,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       // so each stage samples the whole box evenly and the eight sub-tile
       // lists come out about equally long: every lane runs the longest one.
       const int nst = (ncand + 63) >> 6;
+      // piece of this lane's candidate: a binary search for the first
+      // stage, then (candidate u + 1 after u, every piece non-empty) at most
+      // one step forward per stage
+      int lo = 0;
+      {
+        const int u0 = lane * nst;
+        if (u0 < ncand) {
+          int hi = npieces - 1;
+          for (int s = 0; s < nsteps; ++s) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (ts.ppref[mid] <= u0) lo = mid; else hi = mid - 1;
+          }
+        }
+      }
       for (int base = 0; base < ncand; base += 64) {
         // stage 64 candidates
         const int u = lane * nst + (base >> 6);
@@ -788,11 +802,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         uint32_t cpk = 0;
         bool interior = true;
         if (u < ncand) {
-          int lo = 0, hi = npieces - 1;
-          for (int s = 0; s < nsteps; ++s) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (ts.ppref[mid] <= u) lo = mid; else hi = mid - 1;
-          }
+          if (ts.ppref[lo + 1] <= u) ++lo;
           const uint32_t j = ts.pst[lo] + (uint32_t)(u - ts.ppref[lo]);
           int ix, iy;
           if (cand_image(ts.ptx[lo], ts.pty[lo], WR[j], ix, iy)) {
