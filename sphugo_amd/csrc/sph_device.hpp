@@ -84,16 +84,18 @@ __device__ inline uint32_t img_order(uint32_t code) {
 }
 
 // Per-particle record gathered by the force kernel (written by the kNN
-// kernel once density, sound speed and h are known).
+// kernel once density, sound speed and h are known).  rho, c and h are
+// stored halved: the pair means 0.5 (a_i + a_j) are then one add, and equal
+// (a_i / 2 + a_j / 2 rounds as (a_i + a_j) / 2: halving is exact).
 struct alignas(16) RecD {
-  double x, y, vpx, vpy, rho, c, h, B;  // B = c^2 / (gamma rho)
+  double x, y, vpx, vpy, rho, c, h, B;  // rho, c, h halved; B = c^2 / (gamma rho)
 };
 // FP32C record: the position as 32-bit fixed point of the box (units of
 // L / 2^32, wrapped modulo 2^32), so that the 32-bit difference of two
 // records is the minimum-image displacement; the rest in FP32.
 struct alignas(16) RecF {
   uint32_t ux, uy;
-  float vpx, vpy, rho, c, h, B;
+  float vpx, vpy, rho, c, h, B;  // rho, c, h halved, as in RecD
 };
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));

@@ -569,9 +569,9 @@ __device__ inline void write_result(const SimDev* S, int64_t t, double xq, doubl
     r.uy = (uint32_t)(int64_t)floor((yq - S->y0) * (4294967296.0 / S->ly));
     r.vpx = (float)VPX[t];
     r.vpy = (float)VPY[t];
-    r.rho = (float)rho;
-    r.c = (float)c;
-    r.h = (float)h;
+    r.rho = (float)(0.5 * rho);
+    r.c = (float)(0.5 * c);
+    r.h = (float)(0.5 * h);
     r.B = (float)B;
     ((RecF*)recp)[t] = r;
   } else {
@@ -580,9 +580,9 @@ __device__ inline void write_result(const SimDev* S, int64_t t, double xq, doubl
     r.y = yq;
     r.vpx = VPX[t];
     r.vpy = VPY[t];
-    r.rho = rho;
-    r.c = c;
-    r.h = h;
+    r.rho = 0.5 * rho;
+    r.c = 0.5 * c;
+    r.h = 0.5 * h;
     r.B = B;
     ((RecD*)recp)[t] = r;
   }
@@ -1217,7 +1217,8 @@ __global__ __launch_bounds__(256) void k_force(
   if constexpr (!FP32C) {
     const RecD* rec = (const RecD*)recp;
     const RecD me = rec[t];
-    xi = me.x; yi = me.y; hi = me.h; ci = me.c; rhoi = me.rho; A = me.B;
+    xi = me.x; yi = me.y; hi = 2.0 * me.h; ci = 2.0 * me.c; rhoi = 2.0 * me.rho; A = me.B;
+    const double hih = me.h, cih = me.c, rhoih = me.rho;  // halves, for the pair means
     const double vpxi = me.vpx, vpyi = me.vpy;
     const double alpha = S->alpha, beta = S->beta, eta2 = S->eta2;
     const double hinv = EXACT ? 0.0 : 1.0 / hi;
@@ -1245,9 +1246,9 @@ __global__ __launch_bounds__(256) void k_force(
           const double dR = kern_DF(kern, q);
           double piAB = 0.0;
           if (dot < 0) {
-            const double cAB = 0.5 * (ci + nb.c);
-            const double rhoAB = 0.5 * (rhoi + nb.rho);
-            const double hAB = 0.5 * (hi + nb.h);
+            const double cAB = cih + nb.c;
+            const double rhoAB = rhoih + nb.rho;
+            const double hAB = hih + nb.h;
             const double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + eta2);
             piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
           }
@@ -1260,10 +1261,10 @@ __global__ __launch_bounds__(256) void k_force(
           const double dR = kern_DF(kern, q);
           double piAB = 0.0;
           if (dot < 0) {
-            const double cAB = 0.5 * (ci + nb.c);
-            const double hAB = 0.5 * (hi + nb.h);
+            const double cAB = cih + nb.c;
+            const double hAB = hih + nb.h;
             const double muAB = dot * hAB * rcp64((rABx * rABx + rABy * rABy) + eta2);
-            piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * rcp64(0.5 * (rhoi + nb.rho));
+            piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * rcp64(rhoih + nb.rho);
           }
           const double w = (piAB + A + nb.B) * dR * rinv;
           sx += rABx * w;
@@ -1277,20 +1278,21 @@ __global__ __launch_bounds__(256) void k_force(
     const RecF me = rec[t];
     const double2 pme = XY[t];
     xi = pme.x; yi = pme.y;
-    hi = (double)me.h;
+    hi = 2.0 * (double)me.h;
     // displacements from the fixed-point difference (the minimum image) hold
     // while every neighbour lies within half a box; otherwise from FP64
     const bool fixed = hi < 0.45 * (lx < ly ? lx : ly);
     const float fsx = (float)(lx * 0x1p-32), fsy = (float)(ly * 0x1p-32);
-    ci = (double)me.c;
-    rhoi = (double)me.rho;
+    ci = 2.0 * (double)me.c;
+    rhoi = 2.0 * (double)me.rho;
     A = (double)me.B;
+    // own rho, c, h halved as stored (the pair means are sums of halves)
     const float vpxi = me.vpx, vpyi = me.vpy, cif = me.c, rhoif = me.rho, hif = me.h, Af = me.B;
-    const float hinv = __builtin_amdgcn_rcpf(hif);
+    const float hinv = __builtin_amdgcn_rcpf(2.0f * hif);
     const float alpha = (float)S->alpha, beta = (float)S->beta, eta2 = (float)S->eta2;
     // Kahan-compensated sums
     float fx = 0.f, fy = 0.f, fe = 0.f, cx = 0.f, cy = 0.f, ce = 0.f;
-    // 8 records in flight per lane (48 B each): more memory-level
+    // 8 records in flight per lane (32 B each): more memory-level
     // parallelism pays more than the occupancy it costs
     for (int n4 = 0; n4 < k; n4 += 8) {
       const u32x4 q4 = __builtin_nontemporal_load(row.chunk(n4 >> 2));
@@ -1327,9 +1329,9 @@ __global__ __launch_bounds__(256) void k_force(
         const float dot = vx_ * rx + vy_ * ry;
         float piAB = 0.f;
         if (dot < 0.f) {
-          const float cAB = 0.5f * (cif + nb.c);
-          const float rhoAB = 0.5f * (rhoif + nb.rho);
-          const float hAB = 0.5f * (hif + nb.h);
+          const float cAB = cif + nb.c;
+          const float rhoAB = rhoif + nb.rho;
+          const float hAB = hif + nb.h;
           const float muAB = dot * hAB * __builtin_amdgcn_rcpf((rx * rx + ry * ry) + eta2);
           piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * __builtin_amdgcn_rcpf(rhoAB);
         }
