@@ -108,13 +108,23 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   const int64_t base = (int64_t)b * kSortTile;
   const int count = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  // all of the tile's keys and values in flight at once (one memory latency
+  // per block instead of one per round)
+  uint32_t kreg[kSortSub], vreg[kSortSub];
+#pragma unroll
+  for (int r = 0; r < kSortSub; ++r) {
+    const int li = r * kSortThreads + tid;
+    const int64_t i = base + li;
+    kreg[r] = li < count ? kin[i] : 0u;
+    vreg[r] = li < count ? (identity_vals ? (uint32_t)i : vin[i]) : 0u;
+  }
   // stable ranking, 256 keys per round: the keys land in LDS in digit order
+#pragma unroll
   for (int r = 0; r < kSortSub; ++r) {
     const int li = r * kSortThreads + tid;
     const bool valid = li < count;
-    const int64_t i = base + li;
-    const uint32_t key = valid ? kin[i] : 0u;
-    const uint32_t val = valid ? (identity_vals ? (uint32_t)i : vin[i]) : 0u;
+    const uint32_t key = kreg[r];
+    const uint32_t val = vreg[r];
     const uint32_t d = valid ? ((key >> shift) & 255u) : 256u;
     unsigned long long peers = ~0ull;
 #pragma unroll
