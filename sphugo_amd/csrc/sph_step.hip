@@ -1448,6 +1448,9 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
   double acc[kNumRed];
 #pragma unroll
   for (int r = 0; r < kNumRed; ++r) acc[r] = 0.0;
+  // unrolled so that many items' loads are in flight (one block reduces a
+  // whole simulation); the additions keep their order
+#pragma unroll 8
   for (int i = tid; i < ni; i += 256)
 #pragma unroll
     for (int r = 0; r < kNumRed; ++r) acc[r] += partials[(int64_t)(i0 + i) * kNumRed + r];
