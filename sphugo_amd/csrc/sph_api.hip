@@ -167,7 +167,7 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
     if ((rc = h->alloc(&f.id, np))) return rc;
   }
   if ((rc = h->alloc(&h->P[0].x, np)) || (rc = h->alloc(&h->P[0].y, np)) ||
-      (rc = h->alloc(&pb.xyp, np)))
+      (rc = h->alloc(&pb.xyp, np)) || (rc = h->alloc(&pb.uxy, np)))
     return rc;
   if ((rc = h->alloc(&pb.wrap, np)) || (rc = h->alloc(&pb.cstart, h->cell_capacity)) ||
       (rc = h->alloc(&pb.nbr, nbr_entries(np, pb.kmax))) ||

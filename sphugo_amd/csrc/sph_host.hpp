@@ -53,6 +53,7 @@ struct PassBuffers {
   int key_bits;
   Positions pos_cur;    // positions at rest (P0)
   double2* xyp;         // positions of the pass, interleaved (x, y)
+  uint2* uxy;           // FP32C: the same as 32-bit fixed point of the box
   Fields fcur;          // fields, current order
   Fields fnew;          // fields, order of this pass
   uint8_t* wrap;        // wrap code per particle of the pass

@@ -91,8 +91,8 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
 __global__ __launch_bounds__(kItem) void k_permute(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys, int mode,
-    Positions pin, Fields fin, double2* __restrict__ xyp, Fields fout, uint8_t* __restrict__ wrap,
-    uint32_t* __restrict__ cstart) {
+    Positions pin, Fields fin, double2* __restrict__ xyp, uint2* __restrict__ uxy, Fields fout,
+    uint8_t* __restrict__ wrap, uint32_t* __restrict__ cstart) {
   const Item it = items[xcd_item(nitems)];
   if (it.sim < 0) return;  // unused pool slot: stays in place, owns no cells
   const SimDev* S = sims + it.sim;
@@ -149,6 +149,11 @@ __global__ __launch_bounds__(kItem) void k_permute(
     vpx = fin.vpx[i]; vpy = fin.vpy[i]; ep = fin.ep[i];
   }
   xyp[t] = make_double2(x, y);
+  // FP32C: the fixed-point form of the force record (units of L / 2^32,
+  // wrapped), whose 32-bit differences are minimum-image displacements
+  if (S->prec == kPrecFP32C)
+    uxy[t] = make_uint2((uint32_t)(int64_t)floor((x - S->x0) * (4294967296.0 / S->lx)),
+                        (uint32_t)(int64_t)floor((y - S->y0) * (4294967296.0 / S->ly)));
   fout.vx[t] = vx;
   fout.vy[t] = vy;
   fout.e[t] = e;
@@ -998,16 +1003,42 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   }
 }
 
+// Visits the k entries of a neighbour row, 8 at a time: the 8 sources
+// (src[index]) are gathered together and the next 8 entries' row chunks are
+// loaded before the current 8 are processed, so that the row and gather
+// latencies overlap.  Entries past k gather the query's own slot `self`.
+template <typename T, typename Body>
+__device__ inline void row_visit8(const NbrRow row, int k, uint32_t self,
+                                  const T* __restrict__ src, Body body) {
+  u32x4 qa = *row.chunk(0);
+  u32x4 qb = k > 4 ? *row.chunk(1) : qa;
+  for (int n0 = 0; n0 < k; n0 += 8) {
+    const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+    T pj[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) pj[u] = src[n0 + u < k ? (pks[u] >> 4) : self];
+    if (n0 + 8 < k) {
+      qa = *row.chunk((n0 >> 2) + 2);
+      qb = n0 + 12 < k ? *row.chunk((n0 >> 2) + 3) : qa;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (n0 + u >= k) break;
+      body(pks[u], pj[u]);
+    }
+  }
+}
+
 // Completes the fast-path queries: the r = k - c_lo smallest annulus
 // candidates by (d2, ID, image) fill the row, then density, sound speed and
 // the force record.  One thread per query.
 template <bool FP32C>
 __global__ __launch_bounds__(256) void k_knn_finish(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
-    const double2* __restrict__ XY, double* __restrict__ H, const double* __restrict__ EP,
-    const double* __restrict__ VPX, const double* __restrict__ VPY,
-    const uint32_t* __restrict__ ID, uint32_t* __restrict__ nbr, int kstride,
-    double* __restrict__ RHO, double* __restrict__ C, void* __restrict__ recp,
+    const double2* __restrict__ XY, const uint2* __restrict__ UXY, double* __restrict__ H,
+    const double* __restrict__ EP, const double* __restrict__ VPX,
+    const double* __restrict__ VPY, const uint32_t* __restrict__ ID, uint32_t* __restrict__ nbr,
+    int kstride, double* __restrict__ RHO, double* __restrict__ C, void* __restrict__ recp,
     const uint32_t* __restrict__ cnt, const uint32_t* __restrict__ ann,
     KnnStats* __restrict__ stats) {
   const Item it = items[xcd_item(nitems)];
@@ -1024,23 +1055,40 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     const int r = k - c_lo;
     const double2 pq = XY[t];
     const double xq = pq.x, yq = pq.y;
-    const double xqm = xq - lx, xqp = xq + lx, yqm = yq - ly, yqp = yq + ly;
     const NbrRow row = nbr_row(nbr, t, kstride);
-    // annulus squared distances, recomputed exactly as in the walk
+    // annulus squared distances, recomputed exactly as in the walk; slots
+    // from c_an on stand for the query itself at infinite distance, after
+    // every entry.  Every query owns kAnn annulus slots, so FP32C reads all
+    // of them unconditionally (no load waits behind a branch; the unused
+    // ones hold stale values); FP64, whose density loop holds more registers,
+    // keeps the occupancy of reading only the c_an it has.
     const uint32_t* ai = ann + (t >> 6) * (int64_t)(64 * kAnn) + (t & 63);  // stride 64
+    const uint32_t self_pk = ((uint32_t)t << 4) | 5u;
     uint32_t ap[kAnn];
     double ad[kAnn];
 #pragma unroll
-    for (int e = 0; e < kAnn; ++e) ap[e] = e < c_an ? ai[64 * e] : (uint32_t)(t << 4) | 5u;
+    for (int e = 0; e < kAnn; ++e) {
+      if constexpr (FP32C) {
+        const uint32_t raw = ai[64 * e];
+        ap[e] = e < c_an ? raw : self_pk;
+      } else {
+        ap[e] = e < c_an ? ai[64 * e] : self_pk;
+      }
+    }
 #pragma unroll
     for (int e = 0; e < kAnn; ++e) {
       const uint32_t pk = ap[e];
-      const int ix = img_ix(pk & 15u), iy = img_iy(pk & 15u);
-      const double ax = ix == 0 ? xq : (ix < 0 ? xqm : xqp);
-      const double ay = iy == 0 ? yq : (iy < 0 ? yqm : yqp);
       const double2 pj = XY[pk >> 4];
+      // p_i + (-L) rounds as the walk's p_i - L, and p_i + 0 is p_i
+      const double ax = xq + (double)img_ix(pk & 15u) * lx;
+      const double ay = yq + (double)img_iy(pk & 15u) * ly;
       const double ex = ax - pj.x, ey = ay - pj.y;
-      ad[e] = e < c_an ? ex * ex + ey * ey : INFINITY;
+      if constexpr (FP32C) {
+        const double d2 = ex * ex + ey * ey;
+        ad[e] = e < c_an ? d2 : INFINITY;
+      } else {
+        ad[e] = e < c_an ? ex * ex + ey * ey : INFINITY;
+      }
     }
     // the r smallest by (d2, slot): entry e is taken when fewer than r
     // entries precede it in that order, and goes to row position c_lo plus
@@ -1090,42 +1138,52 @@ __global__ __launch_bounds__(256) void k_knn_finish(
         first = false;
       }
     }
-    // density: the row holds the k neighbours; fetch them 8 at a time
+    // density over the row's k neighbours
     const double h = sqrt(tau);
     const int kern = S->kernel;
-    double acc = 0.0;
-    float s_f = 0.f, c_f = 0.f;
-    const float hinv = 1.0f / (float)h;
-    const double hinv64 = 1.0 / h;
-    for (int n0 = 0; n0 < k; n0 += 8) {
-      const u32x4 qa = *row.chunk(n0 >> 2);
-      const u32x4 qb = n0 + 4 < k ? *row.chunk((n0 >> 2) + 1) : qa;
-      const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-      double2 pj[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) pj[u] = XY[n0 + u < k ? (pks[u] >> 4) : (uint32_t)t];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (n0 + u >= k) break;
-        const uint32_t pk = pks[u];
-        const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-        const double ex = (xq + ox) - pj[u].x, ey = (yq + oy) - pj[u].y;
-        if constexpr (FP32C) {
-          // hardware square root (within 1 ulp): the FP32 form's error budget
-          // holds it, and it saves the correctly rounded sequence per pair
-          const float q = __builtin_amdgcn_sqrtf((float)(ex * ex + ey * ey)) * hinv;
-          const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
-          const float y = f - c_f;
-          const float tt = s_f + y;
-          c_f = (tt - s_f) - y;
-          s_f = tt;
-        } else {
-          const double d2 = ex * ex + ey * ey;
-          acc += kern_F(kern, (d2 * rsq64(d2)) * hinv64);
-        }
+    double facc;
+    if constexpr (FP32C) {
+      float s_f = 0.f, c_f = 0.f;
+      auto kahan = [&](float q) {
+        const float f = kern_Ff(kern, __builtin_fminf(q, 1.f));
+        const float y = f - c_f;
+        const float tt = s_f + y;
+        c_f = (tt - s_f) - y;
+        s_f = tt;
+      };
+      // displacements from the fixed-point positions (the minimum image,
+      // exact to L / 2^32) while every neighbour lies within half a box,
+      // scaled straight to units of h; otherwise from the FP64 positions.
+      // Square roots in hardware (within 1 ulp): the FP32 form's error
+      // budget holds them.
+      if (h < 0.45 * (lx < ly ? lx : ly)) {
+        const uint2 me = UXY[t];
+        const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
+        row_visit8(row, k, (uint32_t)t, UXY, [&](uint32_t, uint2 pj) {
+          const float ex = (float)(int32_t)(pj.x - me.x) * gx;
+          const float ey = (float)(int32_t)(pj.y - me.y) * gy;
+          kahan(__builtin_amdgcn_sqrtf(__builtin_fmaf(ey, ey, ex * ex)));
+        });
+      } else {
+        const float hinv = 1.0f / (float)h;
+        row_visit8(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
+          const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
+          const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
+          kahan(__builtin_amdgcn_sqrtf((float)(ex * ex + ey * ey)) * hinv);
+        });
       }
+      facc = (double)s_f - (double)c_f;
+    } else {
+      double acc = 0.0;
+      const double hinv64 = 1.0 / h;
+      row_visit8(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
+        const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
+        const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
+        const double d2 = ex * ex + ey * ey;
+        acc += kern_F(kern, (d2 * rsq64(d2)) * hinv64);
+      });
+      facc = acc;
     }
-    const double facc = FP32C ? (double)s_f - (double)c_f : acc;
     write_result<FP32C>(S, t, xq, yq, tau, facc, EP, VPX, VPY, H, RHO, C, recp);
   }
   for (int o = 32; o > 0; o >>= 1) nties += __shfl_xor(nties, o, 64);
@@ -1502,7 +1560,7 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   mark(1, 1);
   mark(2, 0);
   hipLaunchKernelGGL(k_permute, grd_all, blk, 0, st, b.items_all, b.nitems_all, b.sims, perm, skeys,
-                     perm_mode, b.pos_cur, b.fcur, b.xyp, b.fnew, b.wrap, b.cstart);
+                     perm_mode, b.pos_cur, b.fcur, b.xyp, b.uxy, b.fnew, b.wrap, b.cstart);
   mark(2, 1);
   mark(3, 0);
   {
@@ -1527,11 +1585,11 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
     if (fastpath) {
       if (b.prec == kPrecFP32C)
         hipLaunchKernelGGL((k_knn_finish<true>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
-                           b.xyp, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
+                           b.xyp, b.uxy, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
                            b.kmax, b.fnew.rho, b.fnew.c, b.rec, b.knn_cnt, b.knn_ann, b.stats);
       else
         hipLaunchKernelGGL((k_knn_finish<false>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,
-                           b.xyp, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
+                           b.xyp, b.uxy, b.fnew.h, b.fnew.ep, b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr,
                            b.kmax, b.fnew.rho, b.fnew.c, b.rec, b.knn_cnt, b.knn_ann, b.stats);
     }
     mark(7, 1);
