@@ -157,12 +157,13 @@ __device__ inline double kern_DF(int32_t kernel, double q) {
   if (q < 0.5) return (3 * q * q - 2 * q);
   return -(1 - q) * (1 - q);
 }
+// FP32 forms (FP32C), fused multiply-adds where they fit
 __device__ inline float kern_Ff(int32_t kernel, float q) {
   if (kernel == kWendland) {
     float a = 1.f - q, a2 = a * a;
-    return a2 * a2 * (1.f + 4.f * q);
+    return a2 * a2 * __builtin_fmaf(4.f, q, 1.f);
   }
-  if (q < 0.5f) return q * q * (q - 1.f) + (1.f / 6.f);
+  if (q < 0.5f) return __builtin_fmaf(q * q, q - 1.f, 1.f / 6.f);
   float a = 1.f - q;
   return a * a * a * (1.f / 3.f);
 }
@@ -171,7 +172,7 @@ __device__ inline float kern_DFf(int32_t kernel, float q) {
     float a = 1.f - q;
     return -10.f * q * a * a * a;
   }
-  if (q < 0.5f) return q * (3.f * q - 2.f);
+  if (q < 0.5f) return q * __builtin_fmaf(3.f, q, -2.f);
   float a = 1.f - q;
   return -a * a;
 }

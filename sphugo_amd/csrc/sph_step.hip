@@ -1007,19 +1007,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // (src[index]) are gathered together and the next 8 entries' row chunks are
 // loaded before the current 8 are processed, so that the row and gather
 // latencies overlap.  Entries past k gather the query's own slot `self`.
-template <typename T, typename Body>
+// NT: the row is read for the last time (non-temporal loads).
+template <bool NT = false, typename T, typename Body>
 __device__ inline void row_visit8(const NbrRow row, int k, uint32_t self,
                                   const T* __restrict__ src, Body body) {
-  u32x4 qa = *row.chunk(0);
-  u32x4 qb = k > 4 ? *row.chunk(1) : qa;
+  auto ld = [&](int c) {
+    if constexpr (NT) return __builtin_nontemporal_load(row.chunk(c));
+    else return *row.chunk(c);
+  };
+  // entries past k are never read from the chunks (they gather `self`), so a
+  // chunk past the row is zeros rather than a copy that waits for a load
+  const u32x4 zero = {0u, 0u, 0u, 0u};
+  u32x4 qa = ld(0);
+  u32x4 qb = k > 4 ? ld(1) : zero;
   for (int n0 = 0; n0 < k; n0 += 8) {
     const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
     T pj[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) pj[u] = src[n0 + u < k ? (pks[u] >> 4) : self];
     if (n0 + 8 < k) {
-      qa = *row.chunk((n0 >> 2) + 2);
-      qb = n0 + 12 < k ? *row.chunk((n0 >> 2) + 3) : qa;
+      qa = ld((n0 >> 2) + 2);
+      qb = n0 + 12 < k ? ld((n0 >> 2) + 3) : zero;
     }
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
@@ -1160,6 +1168,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
         const uint2 me = UXY[t];
         const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
         row_visit8(row, k, (uint32_t)t, UXY, [&](uint32_t, uint2 pj) {
+#pragma clang fp contract(fast)
           const float ex = (float)(int32_t)(pj.x - me.x) * gx;
           const float ey = (float)(int32_t)(pj.y - me.y) * gy;
           kahan(__builtin_amdgcn_sqrtf(__builtin_fmaf(ey, ey, ex * ex)));
@@ -1350,20 +1359,49 @@ __global__ __launch_bounds__(256) void k_force(
     const float alpha = (float)S->alpha, beta = (float)S->beta, eta2 = (float)S->eta2;
     // Kahan-compensated sums
     float fx = 0.f, fy = 0.f, fe = 0.f, cx = 0.f, cy = 0.f, ce = 0.f;
-    // 8 records in flight per lane (32 B each): more memory-level
-    // parallelism pays more than the occupancy it costs
-    for (int n4 = 0; n4 < k; n4 += 8) {
-      const u32x4 q4 = __builtin_nontemporal_load(row.chunk(n4 >> 2));
-      const u32x4 q5 = n4 + 4 < k ? __builtin_nontemporal_load(row.chunk((n4 >> 2) + 1)) : q4;
-      const uint32_t pks[8] = {q4.x, q4.y, q4.z, q4.w, q5.x, q5.y, q5.z, q5.w};
-      RecF nbs[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) nbs[u] = rec[n4 + u < k ? (pks[u] >> 4) : (uint32_t)t];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (n4 + u >= k) break;
-        const uint32_t pk = pks[u];
-        const RecF& nb = nbs[u];
+    auto pair = [&](float rx, float ry, const RecF& nb) {
+#pragma clang fp contract(fast)
+      const float d2 = rx * rx + ry * ry;
+      // Two particles closer than the fixed-point step (L / 2^32) have a
+      // zero displacement here; their pair term vanishes with the distance
+      // (DF(q) ~ q and dot ~ r as r -> 0): the reciprocal square root of at
+      // least FLT_MIN keeps it 0 (q = 0, DF(0) = 0, r = 0) rather than the
+      // 0 * inf of rsq(0).
+      const float dinv = __builtin_amdgcn_rsqf(__builtin_fmaxf(d2, 0x1p-126f));
+      const float q = __builtin_fminf((d2 * dinv) * hinv, 1.f);
+      const float dR = kern_DFf(kern, q);
+      const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
+      const float dot = vx_ * rx + vy_ * ry;
+      float piAB = 0.f;
+      if (dot < 0.f) {
+        const float cAB = cif + nb.c;
+        const float rhoAB = rhoif + nb.rho;
+        const float hAB = hif + nb.h;
+        const float muAB = dot * hAB * __builtin_amdgcn_rcpf((rx * rx + ry * ry) + eta2);
+        piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * __builtin_amdgcn_rcpf(rhoAB);
+      }
+      const float w = (piAB + Af + nb.B) * dR * dinv;
+      const float tx = rx * w, ty = ry * w, te = dot * dR;
+      float y_ = tx - cx, tt = fx + y_;
+      cx = (tt - fx) - y_;
+      fx = tt;
+      y_ = ty - cy; tt = fy + y_;
+      cy = (tt - fy) - y_;
+      fy = tt;
+      y_ = te - ce; tt = fe + y_;
+      ce = (tt - fe) - y_;
+      fe = tt;
+    };
+    // 8 records (32 B each) in flight per lane: more memory-level
+    // parallelism pays more than the occupancy it costs.  A wave whose
+    // queries all take fixed-point displacements (the usual case) runs a
+    // loop without the FP64 form, so every record loads whole and at once.
+    if (__all(fixed)) {
+      row_visit8<true>(row, k, (uint32_t)t, rec, [&](uint32_t, const RecF& nb) {
+        pair((float)(int32_t)(nb.ux - me.ux) * fsx, (float)(int32_t)(nb.uy - me.uy) * fsy, nb);
+      });
+    } else {
+      row_visit8<true>(row, k, (uint32_t)t, rec, [&](uint32_t pk, const RecF& nb) {
         float rx, ry;
         if (fixed) {
           rx = (float)(int32_t)(nb.ux - me.ux) * fsx;
@@ -1374,37 +1412,8 @@ __global__ __launch_bounds__(256) void k_force(
           rx = (float)((pj.x - ox) - xi);
           ry = (float)((pj.y - oy) - yi);
         }
-        const float d2 = rx * rx + ry * ry;
-        // Two particles closer than the fixed-point step (L / 2^32) have a
-        // zero displacement here; their pair term vanishes with the distance
-        // (DF(q) ~ q and dot ~ r as r -> 0), so it is taken as 0 rather than
-        // as the 0 * inf of the reciprocal square root.
-        const float dinv = d2 > 0.f ? __builtin_amdgcn_rsqf(d2) : 0.f;
-        float q = (d2 * dinv) * hinv;
-        q = q > 1.f ? 1.f : q;
-        const float dR = kern_DFf(kern, q);
-        const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
-        const float dot = vx_ * rx + vy_ * ry;
-        float piAB = 0.f;
-        if (dot < 0.f) {
-          const float cAB = cif + nb.c;
-          const float rhoAB = rhoif + nb.rho;
-          const float hAB = hif + nb.h;
-          const float muAB = dot * hAB * __builtin_amdgcn_rcpf((rx * rx + ry * ry) + eta2);
-          piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * __builtin_amdgcn_rcpf(rhoAB);
-        }
-        const float w = (piAB + Af + nb.B) * dR * dinv;
-        const float tx = rx * w, ty = ry * w, te = dot * dR;
-        float y_ = tx - cx, tt = fx + y_;
-        cx = (tt - fx) - y_;
-        fx = tt;
-        y_ = ty - cy; tt = fy + y_;
-        cy = (tt - fy) - y_;
-        fy = tt;
-        y_ = te - ce; tt = fe + y_;
-        ce = (tt - fe) - y_;
-        fe = tt;
-      }
+        pair(rx, ry, nb);
+      });
     }
     sx = (double)fx - (double)cx;
     sy = (double)fy - (double)cy;
