@@ -1003,14 +1003,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
   }
 }
 
-// Visits the k entries of a neighbour row, 8 at a time: the 8 sources
-// (src[index]) are gathered together and the next 8 entries' row chunks are
-// loaded before the current 8 are processed, so that the row and gather
-// latencies overlap.  Entries past k gather the query's own slot `self`.
-// NT: the row is read for the last time (non-temporal loads).
-template <bool NT = false, typename T, typename Body>
-__device__ inline void row_visit8(const NbrRow row, int k, uint32_t self,
-                                  const T* __restrict__ src, Body body) {
+// Visits the k entries of a neighbour row, N (4 or 8) at a time: the N
+// sources (src[index]) are gathered together and the next N entries' row
+// chunks are loaded before the current N are processed, so that the row and
+// gather latencies overlap.  Entries past k gather the query's own slot
+// `self`.  NT: the row is read for the last time (non-temporal loads).
+template <int N, bool NT = false, typename T, typename Body>
+__device__ inline void row_visit(const NbrRow row, int k, uint32_t self,
+                                 const T* __restrict__ src, Body body) {
+  static_assert(N == 4 || N == 8, "one or two row chunks per round");
   auto ld = [&](int c) {
     if constexpr (NT) return __builtin_nontemporal_load(row.chunk(c));
     else return *row.chunk(c);
@@ -1019,18 +1020,18 @@ __device__ inline void row_visit8(const NbrRow row, int k, uint32_t self,
   // chunk past the row is zeros rather than a copy that waits for a load
   const u32x4 zero = {0u, 0u, 0u, 0u};
   u32x4 qa = ld(0);
-  u32x4 qb = k > 4 ? ld(1) : zero;
-  for (int n0 = 0; n0 < k; n0 += 8) {
+  u32x4 qb = (N == 8 && k > 4) ? ld(1) : zero;
+  for (int n0 = 0; n0 < k; n0 += N) {
     const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
-    T pj[8];
+    T pj[N];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) pj[u] = src[n0 + u < k ? (pks[u] >> 4) : self];
-    if (n0 + 8 < k) {
-      qa = ld((n0 >> 2) + 2);
-      qb = n0 + 12 < k ? ld((n0 >> 2) + 3) : zero;
+    for (int u = 0; u < N; ++u) pj[u] = src[n0 + u < k ? (pks[u] >> 4) : self];
+    if (n0 + N < k) {
+      qa = ld((n0 + N) >> 2);
+      if constexpr (N == 8) qb = n0 + 12 < k ? ld(((n0 + N) >> 2) + 1) : zero;
     }
 #pragma unroll
-    for (int u = 0; u < 8; ++u) {
+    for (int u = 0; u < N; ++u) {
       if (n0 + u >= k) break;
       body(pks[u], pj[u]);
     }
@@ -1167,7 +1168,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
       if (h < 0.45 * (lx < ly ? lx : ly)) {
         const uint2 me = UXY[t];
         const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
-        row_visit8(row, k, (uint32_t)t, UXY, [&](uint32_t, uint2 pj) {
+        row_visit<8>(row, k, (uint32_t)t, UXY, [&](uint32_t, uint2 pj) {
 #pragma clang fp contract(fast)
           const float ex = (float)(int32_t)(pj.x - me.x) * gx;
           const float ey = (float)(int32_t)(pj.y - me.y) * gy;
@@ -1175,7 +1176,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
         });
       } else {
         const float hinv = 1.0f / (float)h;
-        row_visit8(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
+        row_visit<8>(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
           const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
           const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
           kahan(__builtin_amdgcn_sqrtf((float)(ex * ex + ey * ey)) * hinv);
@@ -1185,7 +1186,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     } else {
       double acc = 0.0;
       const double hinv64 = 1.0 / h;
-      row_visit8(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
+      row_visit<8>(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
         const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
         const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
         const double d2 = ex * ex + ey * ey;
@@ -1289,57 +1290,47 @@ __global__ __launch_bounds__(256) void k_force(
     const double vpxi = me.vpx, vpyi = me.vpy;
     const double alpha = S->alpha, beta = S->beta, eta2 = S->eta2;
     const double hinv = EXACT ? 0.0 : 1.0 / hi;
-    for (int n4 = 0; n4 < k; n4 += 4) {
-      const u32x4 q4 = __builtin_nontemporal_load(row.chunk(n4 >> 2));
-      const uint32_t pks[4] = {q4.x, q4.y, q4.z, q4.w};
-      RecD nbs[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) nbs[u] = rec[n4 + u < k ? (pks[u] >> 4) : (uint32_t)t];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (n4 + u >= k) break;
-        const uint32_t pk = pks[u];
-        const RecD& nb = nbs[u];
-        const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-        const double ex = (xi + ox) - nb.x, ey = (yi + oy) - nb.y;
-        const double d2 = ex * ex + ey * ey;
-        const double rABx = (nb.x - ox) - xi, rABy = (nb.y - oy) - yi;
-        const double vABx = nb.vpx - vpxi, vABy = nb.vpy - vpyi;
-        const double dot = vABx * rABx + vABy * rABy;
-        if constexpr (EXACT) {
-          // the reference's arithmetic, operation for operation
-          const double d = sqrt(d2);
-          const double q = d / hi;
-          const double dR = kern_DF(kern, q);
-          double piAB = 0.0;
-          if (dot < 0) {
-            const double cAB = cih + nb.c;
-            const double rhoAB = rhoih + nb.rho;
-            const double hAB = hih + nb.h;
-            const double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + eta2);
-            piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
-          }
-          sx += rABx * (piAB + A + nb.B) * dR / d;
-          sy += rABy * (piAB + A + nb.B) * dR / d;
-          se += dot * dR;
-        } else {
-          const double rinv = rsq64(d2);
-          const double q = (d2 * rinv) * hinv;
-          const double dR = kern_DF(kern, q);
-          double piAB = 0.0;
-          if (dot < 0) {
-            const double cAB = cih + nb.c;
-            const double hAB = hih + nb.h;
-            const double muAB = dot * hAB * rcp64((rABx * rABx + rABy * rABy) + eta2);
-            piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * rcp64(rhoih + nb.rho);
-          }
-          const double w = (piAB + A + nb.B) * dR * rinv;
-          sx += rABx * w;
-          sy += rABy * w;
-          se += dot * dR;
+    // 4 records (64 B each) in flight per lane, the next row chunk behind them
+    row_visit<4, true>(row, k, (uint32_t)t, rec, [&](uint32_t pk, const RecD& nb) {
+      const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+      const double ex = (xi + ox) - nb.x, ey = (yi + oy) - nb.y;
+      const double d2 = ex * ex + ey * ey;
+      const double rABx = (nb.x - ox) - xi, rABy = (nb.y - oy) - yi;
+      const double vABx = nb.vpx - vpxi, vABy = nb.vpy - vpyi;
+      const double dot = vABx * rABx + vABy * rABy;
+      if constexpr (EXACT) {
+        // the reference's arithmetic, operation for operation
+        const double d = sqrt(d2);
+        const double q = d / hi;
+        const double dR = kern_DF(kern, q);
+        double piAB = 0.0;
+        if (dot < 0) {
+          const double cAB = cih + nb.c;
+          const double rhoAB = rhoih + nb.rho;
+          const double hAB = hih + nb.h;
+          const double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + eta2);
+          piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
         }
+        sx += rABx * (piAB + A + nb.B) * dR / d;
+        sy += rABy * (piAB + A + nb.B) * dR / d;
+        se += dot * dR;
+      } else {
+        const double rinv = rsq64(d2);
+        const double q = (d2 * rinv) * hinv;
+        const double dR = kern_DF(kern, q);
+        double piAB = 0.0;
+        if (dot < 0) {
+          const double cAB = cih + nb.c;
+          const double hAB = hih + nb.h;
+          const double muAB = dot * hAB * rcp64((rABx * rABx + rABy * rABy) + eta2);
+          piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * rcp64(rhoih + nb.rho);
+        }
+        const double w = (piAB + A + nb.B) * dR * rinv;
+        sx += rABx * w;
+        sy += rABy * w;
+        se += dot * dR;
       }
-    }
+    });
   } else {
     const RecF* rec = (const RecF*)recp;
     const RecF me = rec[t];
@@ -1397,11 +1388,11 @@ __global__ __launch_bounds__(256) void k_force(
     // queries all take fixed-point displacements (the usual case) runs a
     // loop without the FP64 form, so every record loads whole and at once.
     if (__all(fixed)) {
-      row_visit8<true>(row, k, (uint32_t)t, rec, [&](uint32_t, const RecF& nb) {
+      row_visit<8, true>(row, k, (uint32_t)t, rec, [&](uint32_t, const RecF& nb) {
         pair((float)(int32_t)(nb.ux - me.ux) * fsx, (float)(int32_t)(nb.uy - me.uy) * fsy, nb);
       });
     } else {
-      row_visit8<true>(row, k, (uint32_t)t, rec, [&](uint32_t pk, const RecF& nb) {
+      row_visit<8, true>(row, k, (uint32_t)t, rec, [&](uint32_t pk, const RecF& nb) {
         float rx, ry;
         if (fixed) {
           rx = (float)(int32_t)(nb.ux - me.ux) * fsx;
