@@ -8,10 +8,12 @@
 //
 // Per pass: (1) block histograms of the digit, (2) one exclusive scan over
 // the digit-major [256][blocks] table, (3) a stable scatter in which each
-// wave ranks its 64 keys by matching digits with 9 ballots, waves of a block
-// are ordered through LDS, the block's tile is gathered in digit order in
-// LDS and then written out so that each digit's run is contiguous.  Stability across passes makes the result
-// independent of launch timing, so runs are bitwise reproducible.
+// wave ranks a contiguous quarter of the block's tile, 64 keys at a time, by
+// matching digits with 9 ballots against per-wave digit counts in LDS; the
+// waves' counts are then scanned once, the tile is gathered in digit order
+// in LDS and written out so that each digit's run is contiguous.  Stability
+// across passes makes the result independent of launch timing, so runs are
+// bitwise reproducible.
 #include "sph_device.hpp"
 #include "sph_host.hpp"
 
@@ -72,9 +74,10 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
     uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, int64_t n,
     int shift, const uint32_t* __restrict__ offsets, const uint32_t* __restrict__ tot,
     int nblocks, int identity_vals) {
-  __shared__ uint32_t wcnt[4][256];
-  __shared__ uint32_t woff[4][256];
-  __shared__ uint32_t run[256];
+  constexpr int kWaves = kSortThreads / 64;
+  constexpr int kPerWave = kSortTile / kWaves;  // contiguous keys ranked by one wave
+  __shared__ uint32_t wcnt[kWaves][256];  // per wave: keys of digit d so far
+  __shared__ uint32_t scan[2][256];
   __shared__ uint32_t gofs[256];    // global position of this block's digit-d keys
   __shared__ uint32_t lstart[256];  // their first position in the block's sorted tile
   __shared__ uint32_t skey[kSortTile];
@@ -83,49 +86,50 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
   const int w = tid >> 6;
   const int lane = tid & 63;
   const int b = blockIdx.x;
-  run[tid] = 0;
+#pragma unroll
+  for (int q = 0; q < kWaves; ++q) wcnt[q][tid] = 0;
   // exclusive prefix of the digit totals, this block's row offset, and the
   // exclusive prefix of this block's own digit counts
   {
     const uint32_t t = tot[tid];
     const uint32_t o = offsets[(int64_t)tid * nblocks + b];
     const uint32_t cnt = (b + 1 < nblocks ? offsets[(int64_t)tid * nblocks + b + 1] : t) - o;
-    woff[0][tid] = t;
-    woff[1][tid] = cnt;
+    scan[0][tid] = t;
+    scan[1][tid] = cnt;
     __syncthreads();
     for (int s = 1; s < 256; s <<= 1) {
-      const uint32_t a = tid >= s ? woff[0][tid - s] : 0u;
-      const uint32_t c = tid >= s ? woff[1][tid - s] : 0u;
+      const uint32_t a = tid >= s ? scan[0][tid - s] : 0u;
+      const uint32_t c = tid >= s ? scan[1][tid - s] : 0u;
       __syncthreads();
-      woff[0][tid] += a;
-      woff[1][tid] += c;
+      scan[0][tid] += a;
+      scan[1][tid] += c;
       __syncthreads();
     }
-    gofs[tid] = woff[0][tid] - t + o;
-    lstart[tid] = woff[1][tid] - cnt;
-    __syncthreads();
+    gofs[tid] = scan[0][tid] - t + o;
+    lstart[tid] = scan[1][tid] - cnt;
   }
   const int64_t base = (int64_t)b * kSortTile;
   const int count = (int)((n - base) < kSortTile ? (n - base) : kSortTile);
   const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  // all of the tile's keys and values in flight at once (one memory latency
-  // per block instead of one per round)
-  uint32_t kreg[kSortSub], vreg[kSortSub];
+  // the wave's keys and values, all in flight at once
+  uint32_t kreg[kSortSub], vreg[kSortSub], loc[kSortSub];
 #pragma unroll
   for (int r = 0; r < kSortSub; ++r) {
-    const int li = r * kSortThreads + tid;
+    const int li = w * kPerWave + r * 64 + lane;
     const int64_t i = base + li;
     kreg[r] = li < count ? kin[i] : 0u;
     vreg[r] = li < count ? (identity_vals ? (uint32_t)i : vin[i]) : 0u;
   }
-  // stable ranking, 256 keys per round: the keys land in LDS in digit order
+  // stable ranking within the wave, 64 keys per round in key order: a key's
+  // place among its wave's keys of the same digit is the digit's count so
+  // far plus the equal-digit lanes below it (matched with 9 ballots); the
+  // lowest such lane then advances the count.  One wave's LDS operations
+  // complete in order, so every round reads the count the last one left.
 #pragma unroll
   for (int r = 0; r < kSortSub; ++r) {
-    const int li = r * kSortThreads + tid;
+    const int li = w * kPerWave + r * 64 + lane;
     const bool valid = li < count;
-    const uint32_t key = kreg[r];
-    const uint32_t val = vreg[r];
-    const uint32_t d = valid ? ((key >> shift) & 255u) : 256u;
+    const uint32_t d = valid ? ((kreg[r] >> shift) & 255u) : 256u;
     unsigned long long peers = ~0ull;
 #pragma unroll
     for (int bit = 0; bit < 9; ++bit) {
@@ -133,30 +137,36 @@ __global__ __launch_bounds__(kSortThreads) void k_sort_scatter(
       const unsigned long long bal = ballot(on);
       peers &= on ? bal : ~bal;
     }
-    const int rank = __popcll(peers & lt_mask);
-    const bool leader = (peers & lt_mask) == 0ull;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) wcnt[q][tid] = 0;
-    __syncthreads();
-    if (valid && leader) wcnt[w][d] = (uint32_t)__popcll(peers);
-    __syncthreads();
-    {
-      const uint32_t c0 = wcnt[0][tid], c1 = wcnt[1][tid], c2 = wcnt[2][tid], c3 = wcnt[3][tid];
-      const uint32_t r0 = run[tid];
-      woff[0][tid] = r0;
-      woff[1][tid] = r0 + c0;
-      woff[2][tid] = r0 + c0 + c1;
-      woff[3][tid] = r0 + c0 + c1 + c2;
-      run[tid] = r0 + c0 + c1 + c2 + c3;
-    }
-    __syncthreads();
-    if (valid) {
-      const uint32_t lp = lstart[d] + woff[w][d] + (uint32_t)rank;
-      skey[lp] = key;
-      sval[lp] = val;
-    }
-    __syncthreads();
+    const unsigned long long below = peers & lt_mask;
+    const uint32_t before = valid ? wcnt[w][d] : 0u;
+    loc[r] = before + (uint32_t)__popcll(below);
+    __builtin_amdgcn_wave_barrier();
+    if (valid && below == 0ull) wcnt[w][d] = before + (uint32_t)__popcll(peers);
+    __builtin_amdgcn_wave_barrier();
   }
+  __syncthreads();
+  // digit d of wave q starts after the block's earlier digits and the
+  // digit-d keys of waves 0 .. q-1
+  {
+    uint32_t run = lstart[tid];
+#pragma unroll
+    for (int q = 0; q < kWaves; ++q) {
+      const uint32_t c = wcnt[q][tid];
+      wcnt[q][tid] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortSub; ++r) {
+    const int li = w * kPerWave + r * 64 + lane;
+    if (li < count) {
+      const uint32_t lp = wcnt[w][(kreg[r] >> shift) & 255u] + loc[r];
+      skey[lp] = kreg[r];
+      sval[lp] = vreg[r];
+    }
+  }
+  __syncthreads();
   // write out in tile order: runs of one digit go to consecutive addresses
   for (int li = tid; li < count; li += kSortThreads) {
     const uint32_t key = skey[li];
