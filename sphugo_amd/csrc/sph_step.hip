@@ -337,28 +337,39 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
     int nsteps = 0;
     while ((1 << nsteps) < npieces) ++nsteps;
     cnt = 0;
-    for (int base = 0; base < ncand; base += 64) {
-      const int u = base + lane;
+    // Stage s takes candidate lane * nst + s: each lane walks a run of
+    // consecutive candidates, so its piece is found by a binary search once
+    // and then moves at most one step per stage (every piece is non-empty).
+    const int nst = (ncand + 63) >> 6;
+    int lo = 0;
+    {
+      const int u0 = lane * nst;
+      if (u0 < ncand) {
+        int hi = npieces - 1;
+        for (int s = 0; s < nsteps; ++s) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (sh.ppref[mid] <= u0) lo = mid; else hi = mid - 1;
+        }
+      }
+    }
+    for (int stage = 0; stage < nst; ++stage) {
+      const int u = lane * nst + stage;
       bool acc = false;
       double d2 = 0;
       uint32_t pk = 0;
       if (u < ncand) {
-        int lo = 0, hi = npieces - 1;
-        for (int s = 0; s < nsteps; ++s) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (sh.ppref[mid] <= u) lo = mid; else hi = mid - 1;
-        }
+        if (sh.ppref[lo + 1] <= u) ++lo;
         const uint32_t j = sh.pst[lo] + (uint32_t)(u - sh.ppref[lo]);
+        // the position is loaded beside the wrap code, not after it
         int ix, iy;
-        if ((int64_t)j != t && cand_image(sh.ptx[lo], sh.pty[lo], WR[j], ix, iy)) {
-          const double ox = (double)ix * lx, oy = (double)iy * ly;
-          const double2 pj = XY[j];
-          const double ex = (xq + ox) - pj.x;
-          const double ey = (yq + oy) - pj.y;
-          d2 = ex * ex + ey * ey;
-          acc = d2 <= R2;
-          pk = (j << 4) | img_code(ix, iy);
-        }
+        const bool vis = cand_image(sh.ptx[lo], sh.pty[lo], WR[j], ix, iy);
+        const bool seen = vis && (int64_t)j != t;
+        const double2 pj = XY[j];
+        const double ex = (xq + (double)ix * lx) - pj.x;
+        const double ey = (yq + (double)iy * ly) - pj.y;
+        d2 = ex * ex + ey * ey;
+        acc = seen && d2 <= R2;
+        pk = (j << 4) | img_code(seen ? ix : 0, seen ? iy : 0);
       }
       const unsigned long long m = ballot(acc);
       if (acc) {
