@@ -1,8 +1,9 @@
 """Edge cases of the HIP path against the oracle (GPU): gravity and
 reflection walls, the Wendland kernel over several steps, anisotropic and
 shifted boxes, exact distance ties on lattices, the smallest k and N, a
-batch whose simulations have different boxes, and a pair of particles closer
-than the FP32C force record's fixed-point step."""
+batch whose simulations have different boxes, FP32C neighbourhoods wider
+than half a box, and a pair of particles closer than the FP32C force
+record's fixed-point step."""
 import numpy as np
 import pytest
 
@@ -116,6 +117,28 @@ def test_smallest_k_and_n():
         g, o = _pair(st, k)
         _check_knn(g, o)
         _check_forces(g, o)
+
+
+@pytest.mark.parametrize("n,k", [(10, 64), (52, 32), (104, 64)])
+def test_fp32c_neighbourhoods_beyond_half_a_box(n, k):
+    # Where h reaches 0.45 L the fixed-point difference need not be the
+    # minimum image, and the FP32C density and force take the displacement
+    # from the FP64 positions instead: for every query (n = 10) or for part
+    # of a wave (the others), the latter in the force's mixed-wave loop.  The
+    # second pass runs the bracketed walk and its finish.
+    st = ics.uniform(n, seed=6)
+    g, o = _pair(st, k, precision=L.PREC_FP32C)
+    wide = o.get("h") >= 0.45
+    assert wide.any()
+    if n > 10:
+        assert not wide.all()
+    for _ in range(2):
+        _check_knn(g, o)
+        for f in ("rho", "ax", "ay", "edot"):
+            a, b = g.get(f), o.get(f)
+            assert np.abs(a - b).max() <= 1e-5 * np.abs(b).max(), f
+        g.calculate_forces()
+        o.calculate_forces()
 
 
 def test_fp32c_pair_closer_than_fixed_point_step():
