@@ -1576,7 +1576,10 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   mark(3, 0);
   {
     const bool fastpath = !b.bitwise && b.have_h;
-    const int fb_grid = fastpath ? 1024 : (b.nitems < 4096 ? b.nitems : 4096);
+    // retry lists: four times as many blocks as fit at once, so that blocks
+    // that drew quick searches make room for others (a few % faster than
+    // one resident round)
+    const int fb_grid = fastpath ? 4096 : (b.nitems < 4096 ? b.nitems : 4096);
 #define SPH_KNN(F32)                                                                          \
   hipLaunchKernelGGL((k_knn<false, F32>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,    \
                      b.xyp, b.wrap, b.cstart, b.fnew.h, b.fnew.ep,                              \
