@@ -6,7 +6,8 @@
 //   sort_pairs      stable LSD radix sort of (key, index) (sph_sort.hip)
 //   k_permute       gather the state into cell order, apply the drift-1 /
 //                   predictor step (sim/sph.go:108-117) or the step-0
-//                   initialisation (:96-101), write the cell ranges
+//                   initialisation (:96-101), write the cell ranges and
+//                   (FP32C) the fixed-point positions
 //   k_knn           tile walk: a wavefront takes 64 consecutive queries,
 //                   stages the candidates of their cells through LDS and
 //                   sorts them against each query's bracket around its last
@@ -27,7 +28,8 @@
 //
 // All FP64 arithmetic on the parity path is evaluated in the reference's
 // association order; the library is compiled with -ffp-contract=off so no
-// product is fused into an add.
+// product is fused into an add there.  Only the FP32C pair terms enable
+// contraction, block by block.
 #include "sph_device.hpp"
 #include "sph_host.hpp"
 
