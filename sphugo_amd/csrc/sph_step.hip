@@ -1098,6 +1098,21 @@ __global__ __launch_bounds__(256) void k_knn_finish(
   const bool valid = (int)threadIdx.x < it.count;
   const int64_t t = it.first + (valid ? threadIdx.x : 0);
   const uint32_t cp = cnt[t];
+  // the block's window of positions in LDS for the density sums, slots
+  // [first - 128, first + 384) (most neighbours of Morton-ordered queries)
+  constexpr int kWin = 512;
+  constexpr int kBack = 128;
+  __shared__ double2 wxy[kWin];
+  __shared__ uint2 wuxy[FP32C ? kWin : 1];
+  const int64_t wb = it.first - kBack;
+  {
+    const SimDev* S = sims + it.sim;
+    if constexpr (FP32C) stage_window<uint2, kWin>(wuxy, UXY, wb, S->off, S->off + S->n);
+    else stage_window<double2, kWin>(wxy, XY, wb, S->off, S->off + S->n);
+    __syncthreads();
+  }
+  const WinSrc<double2, kWin> xys{XY, wxy, (uint32_t)wb};
+  const WinSrc<uint2, FP32C ? kWin : 1> uxys{UXY, wuxy, (uint32_t)wb};
   unsigned long long nties = 0;
   if (valid && cp != 0xFFFFFFFFu) {
     const SimDev* S = sims + it.sim;
@@ -1211,7 +1226,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
       if (h < 0.45 * (lx < ly ? lx : ly)) {
         const uint2 me = UXY[t];
         const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
-        row_visit<8>(row, k, (uint32_t)t, UXY, [&](uint32_t, uint2 pj) {
+        row_visit<8>(row, k, (uint32_t)t, uxys, [&](uint32_t, uint2 pj) {
 #pragma clang fp contract(fast)
           const float ex = (float)(int32_t)(pj.x - me.x) * gx;
           const float ey = (float)(int32_t)(pj.y - me.y) * gy;
@@ -1229,7 +1244,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     } else {
       double acc = 0.0;
       const double hinv64 = 1.0 / h;
-      row_visit<8>(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
+      row_visit<8>(row, k, (uint32_t)t, xys, [&](uint32_t pk, double2 pj) {
         const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
         const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
         const double d2 = ex * ex + ey * ey;
