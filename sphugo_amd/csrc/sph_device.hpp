@@ -157,6 +157,17 @@ __device__ inline double kern_DF(int32_t kernel, double q) {
   if (q < 0.5) return (3 * q * q - 2 * q);
   return -(1 - q) * (1 - q);
 }
+// FP64 derivative with fused multiply-adds (the plain FP64 form's force,
+// whose contract is 1e-12 relative; the bitwise form keeps kern_DF)
+__device__ inline double kern_DF_fma(int32_t kernel, double q) {
+  if (kernel == kWendland) {
+    const double a = 1 - q;
+    return -10 * q * (a * a * a);
+  }
+  if (q < 0.5) return q * fma(3.0, q, -2.0);
+  const double a = 1 - q;
+  return -(a * a);
+}
 // FP32 forms (FP32C), fused multiply-adds where they fit
 __device__ inline float kern_Ff(int32_t kernel, float q) {
   if (kernel == kWendland) {

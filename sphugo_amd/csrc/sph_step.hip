@@ -1364,13 +1364,13 @@ __global__ __launch_bounds__(256) void k_force(
     // 4 records (64 B each) in flight per lane, the next row chunk behind them
     row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t pk, const RecD& nb) {
       const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-      const double ex = (xi + ox) - nb.x, ey = (yi + oy) - nb.y;
-      const double d2 = ex * ex + ey * ey;
       const double rABx = (nb.x - ox) - xi, rABy = (nb.y - oy) - yi;
       const double vABx = nb.vpx - vpxi, vABy = nb.vpy - vpyi;
-      const double dot = vABx * rABx + vABy * rABy;
       if constexpr (EXACT) {
         // the reference's arithmetic, operation for operation
+        const double ex = (xi + ox) - nb.x, ey = (yi + oy) - nb.y;
+        const double d2 = ex * ex + ey * ey;
+        const double dot = vABx * rABx + vABy * rABy;
         const double d = sqrt(d2);
         const double q = d / hi;
         const double dR = kern_DF(kern, q);
@@ -1386,14 +1386,20 @@ __global__ __launch_bounds__(256) void k_force(
         sy += rABy * (piAB + A + nb.B) * dR / d;
         se += dot * dR;
       } else {
-        const double rinv = rsq64(d2);
-        const double q = (d2 * rinv) * hinv;
-        const double dR = kern_DF(kern, q);
+#pragma clang fp contract(fast)
+        // within the 1e-12 contract of this form: the distance from r_AB
+        // (one rounding away from the query-side difference) and fused
+        // multiply-adds throughout
+        const double r2 = rABx * rABx + rABy * rABy;
+        const double dot = vABx * rABx + vABy * rABy;
+        const double rinv = rsq64(r2);
+        const double q = (r2 * rinv) * hinv;
+        const double dR = kern_DF_fma(kern, q);
         double piAB = 0.0;
         if (dot < 0) {
           const double cAB = cih + nb.c;
           const double hAB = hih + nb.h;
-          const double muAB = dot * hAB * rcp64((rABx * rABx + rABy * rABy) + eta2);
+          const double muAB = dot * hAB * rcp64(r2 + eta2);
           piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * rcp64(rhoih + nb.rho);
         }
         const double w = (piAB + A + nb.B) * dR * rinv;
