@@ -1445,8 +1445,12 @@ __global__ __launch_bounds__(256) void k_force(
         const float cAB = cif + nb.c;
         const float rhoAB = rhoif + nb.rho;
         const float hAB = hif + nb.h;
-        const float muAB = dot * hAB * __builtin_amdgcn_rcpf((rx * rx + ry * ry) + eta2);
-        piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * __builtin_amdgcn_rcpf(rhoAB);
+        // mu = dot hAB / (r^2 + eta2), pi = mu (beta mu - alpha cAB) / rhoAB,
+        // with one reciprocal of (r^2 + eta2) rhoAB
+        const float a_ = dot * hAB;
+        const float inv = __builtin_amdgcn_rcpf((d2 + eta2) * rhoAB);
+        const float muAB = a_ * inv * rhoAB;
+        piAB = __builtin_fmaf(beta, muAB, -alpha * cAB) * (a_ * inv);
       }
       const float w = (piAB + Af + nb.B) * dR * dinv;
       const float tx = rx * w, ty = ry * w, te = dot * dR;
