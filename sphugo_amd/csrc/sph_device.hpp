@@ -157,6 +157,17 @@ __device__ inline double kern_DF(int32_t kernel, double q) {
   if (q < 0.5) return (3 * q * q - 2 * q);
   return -(1 - q) * (1 - q);
 }
+// FP64 kernel value with fused multiply-adds (the plain FP64 form's density,
+// whose contract is 1e-12 relative; the bitwise form keeps kern_F)
+__device__ inline double kern_F_fma(int32_t kernel, double q) {
+  if (kernel == kWendland) {
+    const double a = 1 - q, a2 = a * a;
+    return a2 * a2 * fma(4.0, q, 1.0);
+  }
+  if (q < 0.5) return fma(q * q, q - 1, 1.0 / 6);
+  const double a = 1 - q;
+  return a * a * a * (1.0 / 3);
+}
 // FP64 derivative with fused multiply-adds (the plain FP64 form's force,
 // whose contract is 1e-12 relative; the bitwise form keeps kern_DF)
 __device__ inline double kern_DF_fma(int32_t kernel, double q) {

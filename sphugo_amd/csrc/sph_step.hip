@@ -1245,10 +1245,11 @@ __global__ __launch_bounds__(256) void k_knn_finish(
       double acc = 0.0;
       const double hinv64 = 1.0 / h;
       row_visit<8>(row, k, (uint32_t)t, xys, [&](uint32_t pk, double2 pj) {
+#pragma clang fp contract(fast)
         const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
         const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
         const double d2 = ex * ex + ey * ey;
-        acc += kern_F(kern, (d2 * rsq64(d2)) * hinv64);
+        acc += kern_F_fma(kern, (d2 * rsq64(d2)) * hinv64);
       });
       facc = acc;
     }
