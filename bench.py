@@ -41,10 +41,11 @@ def algo_bytes(kernel: str, k: int, precision: str = "fp32c") -> int:
     """Algorithmic HBM bytes per particle of one launch: own-particle streams
     read or written once (state 8 B, IDs and keys 4 B), materialised
     neighbour lists 4 B per entry; neighbour gathers count as on-chip reuse."""
-    rec = 48 if precision == "fp32c" else 64
+    rec = 32 if precision == "fp32c" else 64            # force record (RecF / RecD)
+    uxy = 8 if precision == "fp32c" else 0              # fixed-point positions
     return {
         "keys": 32 + 4,                                   # x, y, vx, vy -> key
-        "permute": 4 + 4 + 9 * 8 + 4 + 10 * 8 + 4 + 1,    # perm, key, gather, write
+        "permute": 4 + 4 + 9 * 8 + 4 + 10 * 8 + 4 + 1 + uxy,  # perm, key, gather, write
         "knn_walk": 16 + 8 + 4 * k + 4,                   # xy, h_prev -> list, count
         "knn_finish": 4 + 16 + 3 * 8 + 4 * k + 3 * 8 + rec,  # count, xy, pred -> list in; h, rho, c, record
         "force": 4 * k + 80 + 64,                         # list, own state -> a, edot, v, e, x, y
