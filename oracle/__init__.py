@@ -145,6 +145,11 @@ def partition(x, y, orientation: int, middle: float):
     return int(na), x, y
 
 
+def hypot(p, q):
+    """Go's math.Hypot as restated for the tree's pruning distances."""
+    return lib().orc_hypot(p, q)
+
+
 def kernel_F(kernel, q):
     return lib().orc_kernel_F(kernel, q)
 

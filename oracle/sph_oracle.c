@@ -9,10 +9,13 @@
  * own and evaluates left to right, and so does this file.
  *
  * Parity status of each part (how it is pinned) is recorded in the test
- * modules tests/test_oracle_*.py and tests/test_golden.py.  One function is
- * parity unpinned: orc_hypot (Go's math.Hypot rounding, which only Go could
- * pin); it only steers the faithful tree's pruning, never the exact kNN the
- * device path is checked against.
+ * modules tests/test_oracle_*.py and tests/test_golden.py.  orc_hypot is
+ * held to what the mathematics fixes (tests/test_oracle_hypot.py: exact
+ * triples, symmetry, exact power-of-two scaling, no overflow, Go's special
+ * cases, 2 ulp against extended precision), but its last-bit rounding
+ * against Go's own math.Hypot stays unpinned (only Go could produce those
+ * vectors); it only steers the faithful tree's pruning, never the exact kNN
+ * the device path is checked against.
  */
 #include "sph_oracle.h"
 
