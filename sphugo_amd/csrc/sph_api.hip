@@ -340,7 +340,7 @@ extern "C" int sph_create(const sph_params* p, int64_t n, const double* x, const
 
 extern "C" void sph_destroy(sph_sim* s) {
   if (!s) return;
-  if (s->st) hipStreamSynchronize(s->st);
+  if (s->st) (void)hipStreamSynchronize(s->st);  // teardown
   delete s;
 }
 
@@ -390,11 +390,11 @@ static int graph_step(sph_sim* s) {
     const int rc = enqueue_pass(s->pb, kPermPredict, kForceKick, s->st, nullptr);
     hipError_t e = hipStreamEndCapture(s->st, &graph);
     if (rc || e != hipSuccess) {
-      if (graph) hipGraphDestroy(graph);
+      if (graph) (void)hipGraphDestroy(graph);
       return fail(SPH_EDEVICE, "graph capture failed: %s", hipGetErrorString(e));
     }
     e = hipGraphInstantiate(&s->gexec[g], graph, nullptr, nullptr, 0);
-    hipGraphDestroy(graph);
+    (void)hipGraphDestroy(graph);
     if (e != hipSuccess) return fail(SPH_EDEVICE, "graph instantiate: %s", hipGetErrorString(e));
   }
   HIPCHK(hipGraphLaunch(s->gexec[g], s->st));
@@ -451,23 +451,27 @@ extern "C" int sph_autotune(sph_sim* s, double* best_ms, double* best_cpp) {
     }
     // a pass to set h, then the timed pass
     if ((rc = run_pass(s, kPermInit, kForceNoKick))) break;
-    hipEventRecord(e0, s->st);
+    // a failed record shows in the synchronisation below
+    (void)hipEventRecord(e0, s->st);
     if ((rc = run_pass(s, kPermInit, kForceNoKick))) break;
-    hipEventRecord(e1, s->st);
+    (void)hipEventRecord(e1, s->st);
     if (hipEventSynchronize(e1) != hipSuccess) {
       rc = fail(SPH_EDEVICE, "autotune pass failed");
       break;
     }
     float ms = 0;
-    hipEventElapsedTime(&ms, e0, e1);
+    if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+      rc = fail(SPH_EDEVICE, "autotune timing failed");
+      break;
+    }
     if (ms < best) {
       best = ms;
       bcpp = cpp;
     }
   }
   s->timing = timing;
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   if ((rc = fetch_sims(s))) return rc;
   layout_cells(s, bcpp);

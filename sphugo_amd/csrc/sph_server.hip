@@ -309,7 +309,7 @@ bool tick(sph_server* s) {
   }
   const Clock::time_point t_tick = Clock::now();
   int rc = 0;
-  if (s->ev_tick[0]) hipEventRecord(s->ev_tick[0], c.st);
+  if (s->ev_tick[0]) (void)hipEventRecord(s->ev_tick[0], c.st);  // timing only
   for (int q : s->to_clear)
     if (!rc && !s->slots[q]) rc = clear_slot(s, q);
   s->to_clear.clear();
@@ -414,7 +414,8 @@ bool tick(sph_server* s) {
 }
 
 void worker_main(sph_server* s) {
-  hipSetDevice(s->core.device);
+  // the device was selected and checked when the server was created
+  (void)hipSetDevice(s->core.device);
   while (tick(s)) {
   }
   std::lock_guard<std::mutex> lk(s->mu);
@@ -518,9 +519,9 @@ extern "C" void sph_server_destroy(sph_server* s) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
-  hipStreamSynchronize(s->core.st);
+  (void)hipStreamSynchronize(s->core.st);  // teardown
   for (auto& e : s->ev_tick)
-    if (e) hipEventDestroy(e);
+    if (e) (void)hipEventDestroy(e);
   for (auto& kv : s->jobs) delete kv.second;
   delete s;
 }

@@ -76,16 +76,17 @@ struct sph_sim {
   void drop_graphs() {
     for (auto& g : gexec)
       if (g) {
-        hipGraphExecDestroy(g);
+        (void)hipGraphExecDestroy(g);
         g = nullptr;
       }
   }
   ~sph_sim() {
     drop_graphs();
-    for (void* q : allocs) hipFree(q);
+    // teardown: nothing is left to report a failure to
+    for (void* q : allocs) (void)hipFree(q);
     for (auto& e : ev)
-      if (e) hipEventDestroy(e);
-    if (st) hipStreamDestroy(st);
+      if (e) (void)hipEventDestroy(e);
+    if (st) (void)hipStreamDestroy(st);
   }
   void bind() {
     pb.fcur = F[cur];
