@@ -158,36 +158,6 @@ def cpu_baseline(threads_all: int):
     }
 
 
-SERVE_TEMPLATE_N = 1 << 20
-SERVE_TEMPLATES = 16
-
-
-def serve_ics_factory(seed=5):
-    """Initial conditions of served jobs: 16 clustered templates of 2^20
-    particles (the clustered recipe, template seeds seed*1000+t), shuffled
-    once; job i takes a window of n particles of template t_i from a random
-    start and shifts it by a random periodic offset (all drawn from the job's
-    own seed), so every job has its own clustered layout at a few ms of
-    host time."""
-    base = []
-    for t in range(SERVE_TEMPLATES):
-        st = ics.clustered_pcg(SERVE_TEMPLATE_N, seed=seed * 1000 + t)
-        perm = np.random.default_rng(seed * 1000 + t).permutation(SERVE_TEMPLATE_N)
-        base.append({c: st[c][perm] for c in ("x", "y", "vx", "vy")})
-
-    def make(i, n, k):
-        g = np.random.default_rng(seed * 10**6 + i)
-        b = base[int(g.integers(SERVE_TEMPLATES))]
-        idx = (int(g.integers(SERVE_TEMPLATE_N)) + np.arange(n)) % SERVE_TEMPLATE_N
-        sx, sy = g.random(2)
-        x = ics.wrap_unit(b["x"][idx] + sx)
-        y = ics.wrap_unit(b["y"][idx] + sy)
-        return {"x": x, "y": y, "vx": b["vx"][idx], "vy": b["vy"][idx], "e": np.ones(n),
-                "mass": 1.0 / n, "k": k, "n": n}
-
-    return make
-
-
 def run_serve(njobs, precision, capacity, max_jobs=256, load=0.7):
     """Poisson arrivals of jobs (N log-uniform in [1024, 2^20], k in {32, 64},
     steps uniform in [10, 200]) into the continuous-batching server at `load`
@@ -195,7 +165,7 @@ def run_serve(njobs, precision, capacity, max_jobs=256, load=0.7):
     import queue
     import threading
 
-    make = serve_ics_factory()
+    make = ics.serve_ics(seed=5)
     prec = L.PREC_FP32C if precision == "fp32c" else L.PREC_FP64
 
     def submit(srv, i, n, k, steps):

@@ -147,8 +147,16 @@ int sph_read_field(sph_sim* s, int32_t field, double* out);
 /* Overwrite one field (X..VPY) of every particle, indexed by ID. */
 int sph_write_field(sph_sim* s, int32_t field, const double* in);
 /* Neighbours of the last pass: ids[ID*k + i] and d2[ID*k + i] (d2 may be
- * NULL), rows sorted ascending by (squared distance, ID). */
+ * NULL), rows sorted ascending by (squared distance, ID).  In a batch whose
+ * simulations have different k, rows are k_max long and entries past a
+ * simulation's k read ID 0xFFFFFFFF and d2 NaN. */
 int sph_read_neighbours(sph_sim* s, uint32_t* ids, double* d2);
+/* NNPos of the last pass (Particle.NNPos, sim/core.go:38, filled at
+ * sim/nearest-neighbour.go:81,152): the neighbours' positions seen through the
+ * query's periodic image, p_j - o, in the row order of sph_read_neighbours
+ * (so the reference's NNPos[0], the farthest, is row entry k-1 and NNPos[1]
+ * entry k-2; AnimatorGL.AddFrame reads those two, sim/animator.go:171-172). */
+int sph_read_neighbour_pos(sph_sim* s, double* px, double* py);
 
 /* sum e, sum rho, E = sum m(e + v^2/2), Px, Py, last dt_half (FP64, fixed
  * reduction order). */
@@ -258,7 +266,8 @@ int sph_server_job_info(sph_server* s, int64_t job_id, int32_t* state, double in
 /* One field of a finished job, indexed by ID: SPH_X, SPH_Y, SPH_VX, SPH_VY,
  * SPH_RHO, SPH_E or SPH_H. */
 int sph_server_read_result(sph_server* s, int64_t job_id, int32_t field, double* out);
-/* sph_reductions of a finished job, then TotalMomentum (|v| of ID n-1's slot). */
+/* sph_reductions of a finished job, then TotalMomentum (|v| of the particle
+ * stored last, as sph_total_momentum). */
 int sph_server_job_reductions(sph_server* s, int64_t job_id, double out[7]);
 /* Forget a finished or failed job and its results. */
 int sph_server_release(sph_server* s, int64_t job_id);

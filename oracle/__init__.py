@@ -100,6 +100,8 @@ def lib():
         "orc_set_field": (None, [vp, ctypes.c_int32, _D]),
         "orc_get_neighbours": (ctypes.c_int64, [vp, _I64, _D]),
         "orc_reductions": (None, [vp, _D]),
+        "orc_get_term_scale": (None, [vp, ctypes.c_int32, _D]),
+        "orc_get_neighbour_pos": (None, [vp, _D, _D]),
         "orc_total_energy": (ctypes.c_double, [vp]),
         "orc_total_density": (ctypes.c_double, [vp]),
         "orc_total_momentum": (ctypes.c_double, [vp]),
@@ -245,6 +247,20 @@ class OracleSim:
         d2 = np.zeros((self.n, self.k))
         lib().orc_get_neighbours(self._h, ids.ctypes.data_as(_I64), _dp(d2))
         return ids, d2
+
+    def term_scale(self, name: str) -> np.ndarray:
+        """Per ID, sum over the neighbours of |term| of ax, ay or edot (the
+        size of what the sum cancels), scaled as the sum is."""
+        out = np.zeros(self.n)
+        lib().orc_get_term_scale(self._h, {"ax": 0, "ay": 1, "edot": 2}[name], _dp(out))
+        return out
+
+    def neighbour_pos(self):
+        """NNPos of the last pass, rows ascending by (d2, ID)."""
+        px = np.zeros((self.n, self.k))
+        py = np.zeros((self.n, self.k))
+        lib().orc_get_neighbour_pos(self._h, _dp(px), _dp(py))
+        return px, py
 
     def reductions(self):
         out = np.zeros(6)

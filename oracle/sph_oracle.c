@@ -94,6 +94,10 @@ double orc_kernel_DFpref(int32_t kernel) {
  * particle when the tree permutes the array (as the Go struct fields do). */
 typedef struct {
   double x, y, vx, vy, rho, c, e, edot, ax, ay, ep, vpx, vpy;
+  /* sum over the neighbour terms of |term| for ax, ay and edot, scaled as
+   * the sums are (test measure: error per particle relative to the size of
+   * the terms it cancels; not read by the step) */
+  double sax, say, sedot;
   int64_t id;
 } OP;
 
@@ -753,6 +757,7 @@ static void force_one(orc_sim* s, OP* p) {
   double contributionB = 0.0;
   double dRKernel = 0.0;
   double acc_ax = 0.0, acc_ay = 0.0, acc_edot = 0.0;
+  double abs_ax = 0.0, abs_ay = 0.0, abs_edot = 0.0;
   double alpha = s->p.visc_alpha, beta = s->p.visc_beta, etaSq = s->p.visc_eta2;
   for (int i = 0; i < k; i++) {
     if (I[i] < 0) break; /* nil neighbour, :346-349 */
@@ -771,14 +776,22 @@ static void force_one(orc_sim* s, OP* p) {
       double muAB = dot * hAB / ((rABx * rABx + rABy * rABy) + etaSq);
       piAB = (-alpha * cAB * muAB + beta * muAB * muAB) / rhoAB;
     }
-    acc_ax += rABx * (piAB + contributionA + contributionB) * dRKernel / D[i];
-    acc_ay += rABy * (piAB + contributionA + contributionB) * dRKernel / D[i];
+    const double tx = rABx * (piAB + contributionA + contributionB) * dRKernel / D[i];
+    const double ty = rABy * (piAB + contributionA + contributionB) * dRKernel / D[i];
+    acc_ax += tx;
+    acc_ay += ty;
     acc_edot += dot * dRKernel;
+    abs_ax += fabs(tx);
+    abs_ay += fabs(ty);
+    abs_edot += fabs(dot * dRKernel);
   }
   double f = s->p.mass * orc_kernel_DFpref(s->p.kernel) / (maxR * maxR * maxR);
   p->ax = acc_ax * f + s->p.accel[0];
   p->ay = acc_ay * f + s->p.accel[1];
   p->edot = contributionA * acc_edot * s->p.mass;
+  p->sax = abs_ax * fabs(f);
+  p->say = abs_ay * fabs(f);
+  p->sedot = fabs(contributionA * s->p.mass) * abs_edot;
 }
 
 /* CalculateForces, sim/sph.go:403-435 */
@@ -959,6 +972,51 @@ int64_t orc_get_neighbours(const orc_sim* s, int64_t* ids, double* d2) {
   }
   free(tmp);
   return nil;
+}
+
+void orc_get_term_scale(const orc_sim* s, int32_t which, double* out) {
+  for (int64_t i = 0; i < s->n; i++) {
+    const OP* p = &s->ps[i];
+    out[p->id] = which == 0 ? p->sax : (which == 1 ? p->say : p->sedot);
+  }
+}
+
+/* NNPos (sim/nearest-neighbour.go:81,152, the neighbour's position seen through
+ * the query's image offset, p_j - o), rows in the order of
+ * orc_get_neighbours: ascending by (d2, ID). */
+typedef struct {
+  double d2, id, px, py;
+} NbrPos;
+static int cmp_nbrpos(const void* a, const void* b) {
+  const NbrPos* x = (const NbrPos*)a;
+  const NbrPos* y = (const NbrPos*)b;
+  if (x->d2 != y->d2) return x->d2 < y->d2 ? -1 : 1;
+  if (x->id != y->id) return x->id < y->id ? -1 : 1;
+  return 0;
+}
+
+void orc_get_neighbour_pos(const orc_sim* s, double* px, double* py) {
+  int k = s->k;
+  NbrPos* tmp = (NbrPos*)malloc((size_t)k * sizeof(NbrPos));
+  for (int64_t i = 0; i < s->n; i++) {
+    int64_t id = s->ps[i].id;
+    int m = 0;
+    for (int t = 0; t < k; t++) {
+      int64_t j = s->nni[id * k + t];
+      if (j < 0) continue;
+      tmp[m].d2 = s->nnd2[id * k + t];
+      tmp[m].id = (double)s->ps[j].id;
+      tmp[m].px = s->nnpx[id * k + t];
+      tmp[m].py = s->nnpy[id * k + t];
+      m++;
+    }
+    qsort(tmp, (size_t)m, sizeof(NbrPos), cmp_nbrpos);
+    for (int t = 0; t < k; t++) {
+      px[id * k + t] = t < m ? tmp[t].px : NAN;
+      py[id * k + t] = t < m ? tmp[t].py : NAN;
+    }
+  }
+  free(tmp);
 }
 
 double orc_total_energy(const orc_sim* s) { /* sim/sph.go:441-447 */

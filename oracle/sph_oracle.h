@@ -104,6 +104,12 @@ void orc_set_field(orc_sim* s, int32_t field, const double* in);
 /* neighbour IDs and squared distances of the last pass, indexed by ID,
  * each row sorted ascending by (d2, ID).  Returns number of nil entries. */
 int64_t orc_get_neighbours(const orc_sim* s, int64_t* ids, double* d2);
+/* per ID: sum over the neighbours of |term| of ax (which 0), ay (1) or
+ * edot (2), with the same prefactor as the sum (last force pass) */
+void orc_get_term_scale(const orc_sim* s, int32_t which, double* out);
+/* NNPos of the last pass (p_j minus the query's image offset), rows
+ * ascending by (d2, ID) as orc_get_neighbours */
+void orc_get_neighbour_pos(const orc_sim* s, double* px, double* py);
 /* sum_e, sum_rho, E_tot, Px, Py, last dt_half */
 void orc_reductions(const orc_sim* s, double out[6]);
 /* reference totals (sim/sph.go:441-463) */

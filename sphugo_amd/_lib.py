@@ -68,6 +68,7 @@ SIGNATURES = {
     "sph_read_field": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
     "sph_write_field": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
     "sph_read_neighbours": (ctypes.c_int, [_VP, _U32, _D]),
+    "sph_read_neighbour_pos": (ctypes.c_int, [_VP, _D, _D]),
     "sph_reductions": (ctypes.c_int, [_VP, _D]),
     "sph_total_energy": (ctypes.c_int, [_VP, _D]),
     "sph_total_density": (ctypes.c_int, [_VP, _D]),
@@ -252,6 +253,13 @@ class Sim:
         check(lib().sph_read_neighbours(self._h, ids.ctypes.data_as(_U32),
                                         dptr(d2) if with_d2 else None))
         return ids, d2
+
+    def neighbour_pos(self):
+        """NNPos of the last pass (p_j - o), rows as neighbours()."""
+        px = np.empty((self.n, self.k))
+        py = np.empty((self.n, self.k))
+        check(lib().sph_read_neighbour_pos(self._h, dptr(px), dptr(py)))
+        return px, py
 
     def reductions(self):
         out = np.empty(6)

@@ -63,13 +63,19 @@ int sph::check_params(const sph_params* p, int64_t n) {
     return fail(SPH_EINVAL, "unknown precision %d", p->precision);
   if ((p->flags & SPH_FLAG_BITWISE) && p->precision != SPH_PREC_FP64)
     return fail(SPH_EINVAL, "bitwise reference order needs FP64 precision");
-  if (is_open(p->hor) || is_open(p->vert))
-    return fail(SPH_EINVAL, "open boundaries are not supported by the device path; "
-                            "give periodic Hor/VertPeriodicity");
-  if (!(p->hor[1] > p->hor[0]) || !(p->vert[1] > p->vert[0]) || !std::isfinite(p->hor[1] - p->hor[0]) ||
-      !std::isfinite(p->vert[1] - p->vert[0]))
+  // an axis is open when its lower bound is -MaxFloat64, and then its upper
+  // bound must be +MaxFloat64 (the reference panics otherwise,
+  // sim/nearest-neighbour.go:39-55); an upper bound of +MaxFloat64 under a
+  // finite lower one is refused the same way
+  const bool ox = is_open(p->hor), oy = is_open(p->vert);
+  if (ox != (p->hor[1] == DBL_MAX) || oy != (p->vert[1] == DBL_MAX))
+    return fail(SPH_EINVAL, "cannot have open and periodic boundary on one axis at the same time");
+  if ((!ox && (!(p->hor[1] > p->hor[0]) || !std::isfinite(p->hor[1] - p->hor[0]))) ||
+      (!oy && (!(p->vert[1] > p->vert[0]) || !std::isfinite(p->vert[1] - p->vert[0]))))
     return fail(SPH_EINVAL, "periodic box must have finite positive size");
-  if (9 * (n - 1) < p->k) return fail(SPH_EINVAL, "fewer than k candidates over 9 images");
+  const int64_t images = (ox ? 1 : 3) * (oy ? 1 : 3);
+  if (images * (n - 1) < p->k)
+    return fail(SPH_EINVAL, "fewer than k candidates over the %lld images", (long long)images);
   if (!(p->gamma > 1) && !(p->gamma < 1)) return fail(SPH_EINVAL, "gamma must not be 1");
   if (p->dt_half <= 0 && !(p->cfl > 0)) return fail(SPH_EINVAL, "CFL number must be positive");
   return 0;
@@ -97,10 +103,14 @@ void sph::fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
   d.gx = g;
   d.gy = g;
   d.lg = lg;
-  d.x0 = p->hor[0];
-  d.y0 = p->vert[0];
-  d.lx = p->hor[1] - p->hor[0];
-  d.ly = p->vert[1] - p->vert[0];
+  d.open = (is_open(p->hor) ? 1 : 0) | (is_open(p->vert) ? 2 : 0);
+  // an open axis gets a unit placeholder here: its box is set around the
+  // positions (sph::open_box), on the host for a creation guess and on the
+  // device before every pass
+  d.x0 = (d.open & 1) ? 0.0 : p->hor[0];
+  d.y0 = (d.open & 2) ? 0.0 : p->vert[0];
+  d.lx = (d.open & 1) ? 1.0 : p->hor[1] - p->hor[0];
+  d.ly = (d.open & 2) ? 1.0 : p->vert[1] - p->vert[0];
   d.ivx = g / d.lx;
   d.ivy = g / d.ly;
   d.sx = d.lx / g;
@@ -124,6 +134,20 @@ void sph::fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
   d.dt_fixed = p->dt_half;
   d.dt_half = p->dt_half > 0 ? p->dt_half : 0.0;
   d.r_init = 1.2 * std::sqrt(p->k * d.lx * d.ly / (M_PI * (double)n));
+}
+
+void sph::init_open_box(SimDev& d, const double* x, const double* y, int64_t n) {
+  if (!d.open) return;
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (int64_t i = 0; i < n; ++i) {
+    x0 = std::fmin(x0, x[i]);
+    x1 = std::fmax(x1, x[i]);
+    y0 = std::fmin(y0, y[i]);
+    y1 = std::fmax(y1, y[i]);
+  }
+  open_box(d, x0, x1, y0, y1);
+  // first-pass search radius guess over the occupied area
+  d.r_init = 1.2 * std::sqrt(d.k * d.lx * d.ly / (M_PI * (double)n));
 }
 
 void sph::set_grid(SimDev& d, int lg) {
@@ -230,6 +254,11 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
     SimDev d;
     const int lg = grid_log2(ns[s], 4.0);
     fill_simdev(d, ps[s], ns[s], lg);
+    if (d.open) {
+      if (!xs[s] || !ys[s]) return bail(fail(SPH_EINVAL, "positions must not be null"));
+      init_open_box(d, xs[s], ys[s], ns[s]);
+      h->pb.any_open = 1;
+    }
     d.off = off;
     h->hsims.push_back(d);
     h->hparams.push_back(*ps[s]);
@@ -426,8 +455,12 @@ extern "C" int sph_set_graph(sph_sim* s, int32_t on) {
 }
 
 // Autotuner: times force passes over the candidate grid resolutions on the
-// current state (only before the first step, where they cannot change the
-// trajectory) and keeps the fastest.
+// current state (only before the first step) and keeps the fastest.  The
+// step-0 pass that follows searches every query exactly, as an untuned
+// handle's does.  The grid changes the order in which neighbours are found
+// and so the summation order: results stay bit-identical in the bitwise
+// form and within the 1e-12 (FP64) and 1e-5 (FP32C density) contracts
+// otherwise; neighbour sets and h are the same.
 extern "C" int sph_autotune(sph_sim* s, double* best_ms, double* best_cpp) {
   if (!s) return fail(SPH_EINVAL, "null handle");
   if (s->step != 0) return fail(SPH_ESTATE, "autotune runs before the first step");
@@ -478,6 +511,7 @@ extern "C" int sph_autotune(sph_sim* s, double* best_ms, double* best_cpp) {
   HIPCHK(hipMemcpyAsync(s->pb.sims, s->hsims.data(), s->hsims.size() * sizeof(SimDev),
                         hipMemcpyHostToDevice, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
+  s->pb.have_h = 0;  // step 0 searches exactly, as without tuning
   if (best_ms) *best_ms = best;
   if (best_cpp) *best_cpp = bcpp;
   return check_device_flags(s);
@@ -535,10 +569,27 @@ extern "C" int sph_read_neighbours(sph_sim* s, uint32_t* ids, double* d2) {
   if (!s->d_nids && ((rc = s->alloc(&s->d_nids, total)) || (rc = s->alloc(&s->d_nd2, total))))
     return rc;
   s->bind();
-  rc = enqueue_read_neighbours(s->pb, s->d_nids, s->d_nd2, s->st);
+  rc = enqueue_read_neighbours(s->pb, s->d_nids, s->d_nd2, nullptr, nullptr, s->st);
   if (rc) return fail(SPH_EDEVICE, "neighbour readback launch failed");
   HIPCHK(hipMemcpyAsync(ids, s->d_nids, total * sizeof(uint32_t), hipMemcpyDeviceToHost, s->st));
   if (d2) HIPCHK(hipMemcpyAsync(d2, s->d_nd2, total * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
+}
+
+extern "C" int sph_read_neighbour_pos(sph_sim* s, double* px, double* py) {
+  if (!s || !px || !py) return fail(SPH_EINVAL, "null argument");
+  const int64_t total = s->np * s->pb.kout;
+  int rc = 0;
+  if (!s->d_nids && ((rc = s->alloc(&s->d_nids, total)) || (rc = s->alloc(&s->d_nd2, total))))
+    return rc;
+  if (!s->d_npx && ((rc = s->alloc(&s->d_npx, total)) || (rc = s->alloc(&s->d_npy, total))))
+    return rc;
+  s->bind();
+  rc = enqueue_read_neighbours(s->pb, s->d_nids, s->d_nd2, s->d_npx, s->d_npy, s->st);
+  if (rc) return fail(SPH_EDEVICE, "neighbour readback launch failed");
+  HIPCHK(hipMemcpyAsync(px, s->d_npx, total * sizeof(double), hipMemcpyDeviceToHost, s->st));
+  HIPCHK(hipMemcpyAsync(py, s->d_npy, total * sizeof(double), hipMemcpyDeviceToHost, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
   return 0;
 }
@@ -735,6 +786,40 @@ void ckpt_arrays(sph_sim* s, double* a[kCkptFields]) {
                             f.edot,    f.vpx,     f.vpy, f.ep, f.h, f.rho, f.c};
   for (int i = 0; i < kCkptFields; ++i) a[i] = v[i];
 }
+
+// The fields of a simulation record that follow from its parameters, its
+// size and its grid (everything but what the device updates between steps,
+// and the box and first-pass radius of open axes, which are set around the
+// positions).
+bool same_record(const SimDev& a, const SimDev& b) {
+  bool same = a.off == b.off && a.coff == b.coff && a.n == b.n && a.k == b.k &&
+              a.kernel == b.kernel && a.prec == b.prec && a.flags == b.flags &&
+              a.use_cfl == b.use_cfl && a.gx == b.gx && a.gy == b.gy && a.lg == b.lg &&
+              a.pmode == b.pmode && a.fmode == b.fmode && a.open == b.open;
+  auto eq = [](double u, double v) { return std::memcmp(&u, &v, sizeof u) == 0; };
+  const double* pa[] = {&a.gamma, &a.mass, &a.cfl, &a.gxa, &a.gya, &a.hor0, &a.hor1, &a.vert0,
+                        &a.vert1, &a.refl[0], &a.refl[1], &a.refl[2], &a.refl[3], &a.alpha,
+                        &a.beta, &a.eta2, &a.fpref, &a.dfpref, &a.cfac, &a.dt_fixed};
+  const double* pb[] = {&b.gamma, &b.mass, &b.cfl, &b.gxa, &b.gya, &b.hor0, &b.hor1, &b.vert0,
+                        &b.vert1, &b.refl[0], &b.refl[1], &b.refl[2], &b.refl[3], &b.alpha,
+                        &b.beta, &b.eta2, &b.fpref, &b.dfpref, &b.cfac, &b.dt_fixed};
+  for (size_t i = 0; i < sizeof pa / sizeof pa[0]; ++i) same = same && eq(*pa[i], *pb[i]);
+  if (!(a.open & 1))
+    same = same && eq(a.x0, b.x0) && eq(a.lx, b.lx) && eq(a.ivx, b.ivx) && eq(a.sx, b.sx);
+  if (!(a.open & 2))
+    same = same && eq(a.y0, b.y0) && eq(a.ly, b.ly) && eq(a.ivy, b.ivy) && eq(a.sy, b.sy);
+  if (!a.open) same = same && eq(a.r_init, b.r_init);
+  return same;
+}
+
+// The scalars a device pass updates (half steps, CFL minimum, totals).
+void take_device_scalars(SimDev& d, const SimDev& from) {
+  d.dt_half = from.dt_half;
+  d.dt_last = from.dt_last;
+  d.cfl_min_bits = from.cfl_min_bits;
+  for (int r = 0; r < kNumRed; ++r) d.red[r] = from.red[r];
+  d.last_speed = from.last_speed;
+}
 }  // namespace
 
 extern "C" int sph_checkpoint_size(sph_sim* s, int64_t* bytes) {
@@ -811,8 +896,20 @@ extern "C" int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out
   SimDev sd;
   std::memcpy(&sd, p, sizeof sd);
   p += sizeof sd;
-  if (sd.n != n || sd.off != 0 || sd.k != prm.k)
-    return fail(SPH_EINVAL, "corrupt checkpoint (simulation record)");
+  // The parameters must be valid, and the simulation record must be the one
+  // they give (grid, box, physics, offsets): only the scalars the device
+  // updates between steps are taken from the image.  Checked before any
+  // device call.
+  int rc = check_params(&prm, n);
+  if (rc) return fail(SPH_EINVAL, "corrupt checkpoint (parameters): %s", sph_last_error());
+  {
+    SimDev ref;
+    fill_simdev(ref, &prm, n, grid_log2(n, hd.grid_cpp));
+    if (!same_record(sd, ref))
+      return fail(SPH_EINVAL, "corrupt checkpoint (simulation record does not match its parameters)");
+    if (!std::isfinite(sd.dt_half) || sd.dt_half < 0 || !std::isfinite(sd.dt_last))
+      return fail(SPH_EINVAL, "corrupt checkpoint (half step)");
+  }
   // aligned host copies of what sph_create takes; the rest goes straight up
   std::vector<double> init(5 * n);
   for (int i = 0; i < 5; ++i) std::memcpy(init.data() + i * n, p + i * n * sizeof(double), n * sizeof(double));
@@ -823,16 +920,14 @@ extern "C" int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out
   const uint32_t* idp = ids.data();
   const sph_params* pp = &prm;
   sph_sim* h = nullptr;
-  int rc = create_batch(1, &pp, &n, &xs[0], &xs[1], &xs[2], &xs[3], &xs[4], &idp, &h);
+  rc = create_batch(1, &pp, &n, &xs[0], &xs[1], &xs[2], &xs[3], &xs[4], &idp, &h);
   if (rc) return rc;
   auto bail = [&](int code) {
     sph_destroy(h);
     return code;
   };
   layout_cells(h, hd.grid_cpp);
-  if (h->hsims[0].lg != sd.lg || h->hsims[0].coff != sd.coff)
-    return bail(fail(SPH_EINVAL, "corrupt checkpoint (grid of the simulation record)"));
-  h->hsims[0] = sd;
+  take_device_scalars(h->hsims[0], sd);
   double* a[kCkptFields];
   ckpt_arrays(h, a);
   for (int i = 5; i < kCkptFields; ++i)
@@ -900,13 +995,7 @@ extern "C" int sph_append(sph_sim* s, int64_t m, const double* x, const double* 
   layout_cells(h, s->grid_cpp);  // same cells per particle as before
   // device-updated scalars carry over (the next half step was set by the
   // last pass, over the particles that had neighbours)
-  SimDev& d = h->hsims[0];
-  const SimDev& o = s->hsims[0];
-  d.dt_half = o.dt_half;
-  d.dt_last = o.dt_last;
-  d.cfl_min_bits = o.cfl_min_bits;
-  for (int r = 0; r < kNumRed; ++r) d.red[r] = o.red[r];
-  d.last_speed = o.last_speed;
+  take_device_scalars(h->hsims[0], s->hsims[0]);
   double* dst[kCkptFields];
   ckpt_arrays(h, dst);
   for (int f = 5; f < kCkptFields; ++f)

@@ -36,9 +36,10 @@ struct SimDev {
   int32_t lg;       // log2 of cells per axis (Morton order)
   int32_t pmode;    // pass modes when a launch runs each simulation in its own
   int32_t fmode;    // mode (continuous batching): PermuteMode / ForceMode
-  int32_t pad1;
-  double x0, y0;    // periodic box origin (Hor[0], Vert[0])
-  double lx, ly;    // box size (Hor[1]-Hor[0], Vert[1]-Vert[0])
+  int32_t open;     // open axes (bit 0: x, bit 1: y; bounds +-MaxFloat64)
+  double x0, y0;    // grid box origin: Hor[0], Vert[0] of a periodic axis; on
+  double lx, ly;    // an open axis the box around the pass positions, set on
+                    // the device before each pass (k_box)
   double ivx, ivy;  // cells per unit length
   double sx, sy;    // cell size
   double gamma, mass, cfl;
@@ -57,6 +58,47 @@ struct SimDev {
   double red[kNumRed];  // reductions of the last pass
   double last_speed;    // |v| of the particle stored last
 };
+
+// Grid box of a simulation's open axes around the range [lo, hi] of its pass
+// positions per axis (the reference searches no periodic images on such an
+// axis, sim/nearest-neighbour.go:39-55).  The box is widened by 2^-8 of the
+// range plus 2^-40 of the coordinates' size, far beyond the rounding of the
+// box and of floor((x - x0) * ivx), so every position lands strictly inside
+// cells [0, 2^lg).  With both axes open the box is a square (square cells).
+// An empty range takes the other axis's size (or 1).  Same arithmetic on the
+// host (the creation guess) and the device (each pass).
+__host__ __device__ inline void open_box(SimDev& d, double xlo, double xhi, double ylo,
+                                         double yhi) {
+  const double G = (double)(1 << d.lg);
+  auto pad = [](double lo, double hi) {
+    return (hi - lo) * 0x1p-8 + (fabs(lo) + fabs(hi)) * 0x1p-40 + 0x1p-1000;
+  };
+  const bool ox = (d.open & 1) != 0, oy = (d.open & 2) != 0;
+  double wx = ox ? (xhi > xlo ? (xhi - xlo) + 2 * pad(xlo, xhi) : 0.0) : d.lx;
+  double wy = oy ? (yhi > ylo ? (yhi - ylo) + 2 * pad(ylo, yhi) : 0.0) : d.ly;
+  if (ox && oy) {
+    double w = wx > wy ? wx : wy;
+    if (!(w > 0)) w = 1.0;
+    wx = w;
+    wy = w;
+  } else if (ox && !(wx > 0)) {
+    wx = d.ly;
+  } else if (oy && !(wy > 0)) {
+    wy = d.lx;
+  }
+  if (ox) {
+    d.x0 = 0.5 * (xlo + xhi) - 0.5 * wx;
+    d.lx = wx;
+    d.ivx = G / d.lx;
+    d.sx = d.lx / G;
+  }
+  if (oy) {
+    d.y0 = 0.5 * (ylo + yhi) - 0.5 * wy;
+    d.ly = wy;
+    d.ivy = G / d.ly;
+    d.sy = d.ly / G;
+  }
+}
 
 // Morton (Z-order) code of a cell: bits of x at even, y at odd positions.
 __device__ inline uint32_t spread_bits(uint32_t v) {

@@ -67,6 +67,7 @@ struct PassBuffers {
   uint32_t* knn_cnt;       // per query: certain | annulus << 8, or ~0 (retry)
   uint32_t* knn_ann;       // per query: annulus candidates (kAnn entries)
   int have_h;             // every particle has an h from an earlier pass
+  int any_open;           // some simulation has an open axis (k_bbox, k_box run)
   int* err;             // device error flags: [0] any, [1 + s] simulation s (1: a
                         // particle left the images, 2: kNN found fewer than k)
   SortBuffers sort;
@@ -98,8 +99,10 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
 
 int enqueue_read_field(const PassBuffers& b, int field, int sim, double* dev_out, hipStream_t st);
 int enqueue_write_field(PassBuffers& b, int field, int sim, const double* dev_in, hipStream_t st);
+// neighbour rows by ID (ids, d2 and, if dev_px is not null, the image
+// positions NNPos), ascending by (d2, ID); np * kout entries each
 int enqueue_read_neighbours(const PassBuffers& b, uint32_t* dev_ids, double* dev_d2,
-                            hipStream_t st);
+                            double* dev_px, double* dev_py, hipStream_t st);
 int enqueue_init_sims(PassBuffers& b, hipStream_t st);
 
 }  // namespace sph

@@ -191,6 +191,9 @@ int upload_items(sph_server* s) {
   for (auto& h : s->holes)
     if (h.first < hi) push_items(s->h_items_all, -1, h.first, std::min(h.second, hi - h.first));
   PassBuffers& pb = s->core.pb;
+  pb.any_open = 0;
+  for (int q = 0; q < s->max_jobs; ++q)
+    if (s->slots[q] && s->hsim[q].open) pb.any_open = 1;
   pb.nitems = (int)s->h_items.size();
   pb.nitems_all = (int)s->h_items_all.size();
   pb.np = hi;
@@ -215,6 +218,7 @@ int upload_job(sph_server* s, Job* j) {
   const int64_t n = j->n, o = j->off;
   SimDev& d = s->hsim[j->slot];
   fill_simdev(d, &j->p, n, grid_log2(n, 4.0));
+  init_open_box(d, j->in.data(), j->in.data() + n, n);
   d.off = o;
   d.coff = o * kHoleCpp + 1;
   d.pmode = kPermInit;

@@ -56,6 +56,8 @@ struct sph_sim {
   double* d_scratch = nullptr;     // np doubles for readback
   uint32_t* d_nids = nullptr;      // neighbour readback (lazy)
   double* d_nd2 = nullptr;
+  double* d_npx = nullptr;         // neighbour image positions (lazy)
+  double* d_npy = nullptr;
   KnnStats last_stats = {};
   // graph replay of a regular step, one per field-buffer parity
   bool graph_on = false;
@@ -119,6 +121,8 @@ struct sph_sim {
     std::swap(d_scratch, o.d_scratch);
     std::swap(d_nids, o.d_nids);
     std::swap(d_nd2, o.d_nd2);
+    std::swap(d_npx, o.d_npx);
+    std::swap(d_npy, o.d_npy);
     std::swap(last_stats, o.last_stats);
     std::swap(graph_on, o.graph_on);
     std::swap(gexec, o.gexec);
@@ -131,6 +135,9 @@ namespace sph {
 // shared helpers (sph_api.hip)
 void fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg);
 void set_grid(SimDev& d, int lg);
+// box of the open axes around host positions (creation guess; the device
+// sets it again before every pass) and the first-pass radius guess
+void init_open_box(SimDev& d, const double* x, const double* y, int64_t n);
 int check_params(const sph_params* p, int64_t n);
 int fetch_sims(sph_sim* s);
 int check_device_flags(sph_sim* s);

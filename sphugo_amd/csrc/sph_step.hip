@@ -45,15 +45,139 @@ struct CellIdx {
 };
 
 __device__ inline CellIdx cell_of(const SimDev* S, double x, double y) {
-  const double fx = floor((x - S->x0) * S->ivx);
-  const double fy = floor((y - S->y0) * S->ivy);
+  double fx = floor((x - S->x0) * S->ivx);
+  double fy = floor((y - S->y0) * S->ivy);
   const double lox = -2.0 * S->gx, hix = 3.0 * S->gx - 1;
   const double loy = -2.0 * S->gy, hiy = 3.0 * S->gy - 1;
+  // an open axis has no images: its box (k_box) holds every pass position
+  // inside cells [0, G) with a wide margin; the clamp only pins the index
+  if (S->open & 1) fx = fx < 0 ? 0.0 : (fx > S->gx - 1 ? (double)(S->gx - 1) : fx);
+  if (S->open & 2) fy = fy < 0 ? 0.0 : (fy > S->gy - 1 ? (double)(S->gy - 1) : fy);
   CellIdx c;
   c.ok = (fx >= lox) && (fx <= hix) && (fy >= loy) && (fy <= hiy);
   c.ux = c.ok ? (int64_t)fx : 0;
   c.uy = c.ok ? (int64_t)fy : 0;
   return c;
+}
+
+// Cell span [a, b] of a search at a level with nc cells per axis, limited to
+// the images the reference visits: [-2 nc, 3 nc) on a periodic axis (the
+// -L, 0, +L images of positions that may sit up to a box outside it), the
+// box itself on an open axis (sim/nearest-neighbour.go:39-55).
+__device__ inline void clamp_span(int64_t& a, int64_t& b, int64_t nc, bool open) {
+  const int64_t lo = open ? 0 : -2 * nc, hi = open ? nc - 1 : 3 * nc - 1;
+  if (a < lo) a = lo;
+  if (b > hi) b = hi;
+  if (b < a) b = a - 1;  // empty span
+}
+
+// Box of each open simulation's pass positions, in two launches: per work
+// item the range of the positions k_keys bins (same predictor arithmetic),
+// then per simulation the range over its items and the grid box (open_box).
+// Periodic simulations and unused pool slots are skipped.
+__global__ __launch_bounds__(kItem) void k_bbox(const Item* __restrict__ items, int nitems,
+                                                const SimDev* __restrict__ sims,
+                                                const double* __restrict__ X,
+                                                const double* __restrict__ Y,
+                                                const double* __restrict__ VX,
+                                                const double* __restrict__ VY, int mode,
+                                                double* __restrict__ part) {
+  __shared__ double red[4][4];
+  const int item_idx = blockIdx.x;
+  const Item it = items[item_idx];
+  if (it.sim < 0) return;
+  const SimDev* S = sims + it.sim;
+  if (!S->open) return;
+  if (mode < 0) mode = S->pmode;
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  if ((int)threadIdx.x < it.count) {
+    const int64_t t = it.first + threadIdx.x;
+    double x = X[t], y = Y[t];
+    if (mode == kPermPredict) {
+      const double dt = S->dt_half;
+      x = x + VX[t] * dt;
+      y = y + VY[t] * dt;
+    }
+    x0 = x1 = x;
+    y0 = y1 = y;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, o, 64));
+    x1 = fmax(x1, __shfl_xor(x1, o, 64));
+    y0 = fmin(y0, __shfl_xor(y0, o, 64));
+    y1 = fmax(y1, __shfl_xor(y1, o, 64));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wv][0] = x0;
+    red[wv][1] = x1;
+    red[wv][2] = y0;
+    red[wv][3] = y1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      x0 = fmin(x0, red[w][0]);
+      x1 = fmax(x1, red[w][1]);
+      y0 = fmin(y0, red[w][2]);
+      y1 = fmax(y1, red[w][3]);
+    }
+    double* p = part + (int64_t)item_idx * kNumRed;
+    p[0] = x0;
+    p[1] = x1;
+    p[2] = y0;
+    p[3] = y1;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_box(SimDev* __restrict__ sims,
+                                             const double* __restrict__ part,
+                                             const int* __restrict__ item_first,
+                                             const int* __restrict__ item_count) {
+  __shared__ double red[4][4];
+  SimDev* S = sims + blockIdx.x;
+  if (S->n == 0 || !S->open) return;
+  const int i0 = item_first[blockIdx.x], ni = item_count[blockIdx.x];
+  double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+  for (int i = threadIdx.x; i < ni; i += 256) {
+    const double* p = part + (int64_t)(i0 + i) * kNumRed;
+    x0 = fmin(x0, p[0]);
+    x1 = fmax(x1, p[1]);
+    y0 = fmin(y0, p[2]);
+    y1 = fmax(y1, p[3]);
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    x0 = fmin(x0, __shfl_xor(x0, o, 64));
+    x1 = fmax(x1, __shfl_xor(x1, o, 64));
+    y0 = fmin(y0, __shfl_xor(y0, o, 64));
+    y1 = fmax(y1, __shfl_xor(y1, o, 64));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    red[wv][0] = x0;
+    red[wv][1] = x1;
+    red[wv][2] = y0;
+    red[wv][3] = y1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      x0 = fmin(x0, red[w][0]);
+      x1 = fmax(x1, red[w][1]);
+      y0 = fmin(y0, red[w][2]);
+      y1 = fmax(y1, red[w][3]);
+    }
+    SimDev d = *S;
+    open_box(d, x0, x1, y0, y1);
+    S->x0 = d.x0;
+    S->lx = d.lx;
+    S->ivx = d.ivx;
+    S->sx = d.sx;
+    S->y0 = d.y0;
+    S->ly = d.ly;
+    S->ivy = d.ivy;
+    S->sy = d.sy;
+  }
 }
 
 __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, int nitems,
@@ -299,10 +423,8 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
     const double scx = lx / (double)nc, scy = ly / (double)nc;
     int64_t X0 = (int64_t)floor((xq - S->x0 - Rm) * icx), X1 = (int64_t)floor((xq - S->x0 + Rm) * icx);
     int64_t Y0 = (int64_t)floor((yq - S->y0 - Rm) * icy), Y1 = (int64_t)floor((yq - S->y0 + Rm) * icy);
-    if (X0 < -2 * nc) X0 = -2 * nc;
-    if (Y0 < -2 * nc) Y0 = -2 * nc;
-    if (X1 > 3 * nc - 1) X1 = 3 * nc - 1;
-    if (Y1 > 3 * nc - 1) Y1 = 3 * nc - 1;
+    clamp_span(X0, X1, nc, S->open & 1);
+    clamp_span(Y0, Y1, nc, S->open & 2);
     const int nX = (int)(X1 - X0 + 1), nY = (int)(Y1 - Y0 + 1);
     const int ncell = nX * nY;
     int npieces = 0, ncand = 0;
@@ -485,7 +607,12 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
   const int kern = S->kernel;
   double facc = 0.0;
   float fs = 0.f, fc = 0.f;
-  const float hinv_f = 1.0f / (float)h;
+  // FP32C: q^2 = d^2 / tau formed in FP64 before the FP32 square root.  A
+  // rounded 1 / h shared by every term would move all q together, and
+  // F(q) = (1 - q)^3 / 3 amplifies that by 3q / (1 - q): a particle whose
+  // neighbours all sit near q = 1 (the edge of a dense clump) would lose
+  // 1e-5 of its density.
+  const double itau = 1.0 / tau;
   int emitted = 0;
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
@@ -498,7 +625,7 @@ __device__ SearchOut wave_search(const SimDev* S, int64_t t, double xq, double y
         row[pos] = pkr[s];
         if constexpr (FP32C) {
           // hardware square root (within 1 ulp), as in the walk's finish
-          const float q = __builtin_amdgcn_sqrtf((float)v[s]) * hinv_f;
+          const float q = __builtin_amdgcn_sqrtf((float)(v[s] * itau));
           const float f = kern_Ff(kern, q > 1.f ? 1.f : q);
           const float y = f - fc;
           const float tt = fs + y;
@@ -608,6 +735,19 @@ __device__ inline void write_result(const SimDev* S, int64_t t, double xq, doubl
   }
 }
 
+// FP32C: may the displacements of a query with support h come from the 32-bit
+// fixed-point positions (units of L / 2^32)?  Their 32-bit difference is the
+// minimum image while every neighbour lies within half a box (h < 0.45 L),
+// and its quantisation (< one step per component) stays below 2^-21 h while
+// h > 2^-11 L: q moves by under 7e-7, so a 64-term density sum of F
+// (|F'| <= 1/3) moves by under 1.5e-5 even with every error aligned, 8e-6 of
+// the sum of a uniform neighbourhood (about 1.9).  Otherwise the FP64
+// positions give the displacement.
+__device__ inline bool fixed_point_ok(double h, double lx, double ly) {
+  const double lmin = lx < ly ? lx : ly;
+  return h < 0.45 * lmin && h > 0x1p-11 * lmin;
+}
+
 // First-pass search radius: local density from the occupancy of the
 // query's cell at a level holding a few particles.
 __device__ inline double local_radius(const SimDev* S, const uint32_t* __restrict__ cstart,
@@ -712,10 +852,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
       const double icx = (double)nc / lx, icy = (double)nc / ly;
       int64_t X0 = (int64_t)floor((bx0 - S->x0) * icx), X1 = (int64_t)floor((bx1 - S->x0) * icx);
       int64_t Y0 = (int64_t)floor((by0 - S->y0) * icy), Y1 = (int64_t)floor((by1 - S->y0) * icy);
-      if (X0 < -2 * nc) X0 = -2 * nc;
-      if (Y0 < -2 * nc) Y0 = -2 * nc;
-      if (X1 > 3 * nc - 1) X1 = 3 * nc - 1;
-      if (Y1 > 3 * nc - 1) Y1 = 3 * nc - 1;
+      clamp_span(X0, X1, nc, S->open & 1);
+      clamp_span(Y0, Y1, nc, S->open & 2);
       const int nX = (int)(X1 - X0 + 1), nY = (int)(Y1 - Y0 + 1);
       const int ncell = nX * nY;
       int npieces = 0, ncand = 0;
@@ -1223,7 +1361,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
       // scaled straight to units of h; otherwise from the FP64 positions.
       // Square roots in hardware (within 1 ulp): the FP32 form's error
       // budget holds them.
-      if (h < 0.45 * (lx < ly ? lx : ly)) {
+      if (fixed_point_ok(h, lx, ly)) {
         const uint2 me = UXY[t];
         const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
         row_visit<8>(row, k, (uint32_t)t, uxys, [&](uint32_t, uint2 pj) {
@@ -1233,11 +1371,12 @@ __global__ __launch_bounds__(256) void k_knn_finish(
           kahan(__builtin_amdgcn_sqrtf(__builtin_fmaf(ey, ey, ex * ex)));
         });
       } else {
-        const float hinv = 1.0f / (float)h;
+        // q^2 in FP64 before the FP32 square root (see wave_search)
+        const double itau = 1.0 / tau;
         row_visit<8>(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
           const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
           const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
-          kahan(__builtin_amdgcn_sqrtf((float)(ex * ex + ey * ey)) * hinv);
+          kahan(__builtin_amdgcn_sqrtf((float)((ex * ex + ey * ey) * itau)));
         });
       }
       facc = (double)s_f - (double)c_f;
@@ -1416,8 +1555,9 @@ __global__ __launch_bounds__(256) void k_force(
     xi = pme.x; yi = pme.y;
     hi = 2.0 * (double)me.h;
     // displacements from the fixed-point difference (the minimum image) hold
-    // while every neighbour lies within half a box; otherwise from FP64
-    const bool fixed = hi < 0.45 * (lx < ly ? lx : ly);
+    // while every neighbour lies within half a box and h spans enough steps
+    // of L / 2^32; otherwise from FP64
+    const bool fixed = fixed_point_ok(hi, lx, ly);
     const float fsx = (float)(lx * 0x1p-32), fsy = (float)(ly * 0x1p-32);
     ci = 2.0 * (double)me.c;
     rhoi = 2.0 * (double)me.rho;
@@ -1632,6 +1772,12 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   const dim3 grd(b.nitems);
   const dim3 grd_all(b.nitems_all);
   mark(0, 0);
+  if (b.any_open) {
+    hipLaunchKernelGGL(k_bbox, grd, blk, 0, st, b.items, b.nitems, b.sims, b.pos_cur.x,
+                       b.pos_cur.y, b.fcur.vx, b.fcur.vy, perm_mode, b.partials);
+    hipLaunchKernelGGL(k_box, dim3(b.nsims), dim3(256), 0, st, b.sims, b.partials, b.red_first,
+                       b.red_count);
+  }
   hipLaunchKernelGGL(k_keys, grd_all, blk, 0, st, b.items_all, b.nitems_all, b.sims, b.pos_cur.x,
                      b.pos_cur.y, b.fcur.vx, b.fcur.vy, perm_mode, b.sort.keys, b.err,
                      b.retry_count, b.hole_cpp);
@@ -1761,12 +1907,17 @@ int enqueue_write_field(PassBuffers& b, int field, int sim, const double* dev_in
   return (int)hipGetLastError();
 }
 
-// neighbour rows by ID, IDs of the neighbours, recomputed squared distances
-// (same formula as the search), sorted ascending by (d2, ID)
+// Neighbour rows by ID: IDs of the neighbours, recomputed squared distances
+// (same formula as the search) and, if px is not null, the neighbours'
+// positions seen through the query's image offset o (NNPos = p_j - o,
+// sim/nearest-neighbour.go:81,152), sorted ascending by (d2, ID).  Entries of a
+// row past the simulation's k (batches of mixed k) read ID 0xFFFFFFFF and
+// NaN, so every output entry is written.
 __global__ void k_read_nbrs(const Item* __restrict__ items, int nitems,
                             const SimDev* __restrict__ sims, const double2* __restrict__ XY, Fields f,
                             const uint32_t* __restrict__ nbr, int kstride, int kout,
-                            uint32_t* __restrict__ ids, double* __restrict__ d2o) {
+                            uint32_t* __restrict__ ids, double* __restrict__ d2o,
+                            double* __restrict__ px, double* __restrict__ py) {
   const Item it = items[blockIdx.x];
   if ((int)threadIdx.x >= it.count) return;
   const SimDev* S = sims + it.sim;
@@ -1775,30 +1926,49 @@ __global__ void k_read_nbrs(const Item* __restrict__ items, int nitems,
   const int64_t row = (int64_t)f.id[t] + S->off;
   uint32_t* ro = ids + row * kout;
   double* rd = d2o + row * kout;
+  double* rx = px ? px + row * kout : nullptr;
+  double* ry = py ? py + row * kout : nullptr;
   const double xi = XY[t].x, yi = XY[t].y;
   for (int n = 0; n < k; ++n) {
     const uint32_t pk = nbr_row(nbr, t, kstride)[n];
     const uint32_t j = pk >> 4;
     const double ox = (double)img_ix(pk & 15u) * S->lx;
     const double oy = (double)img_iy(pk & 15u) * S->ly;
-    const double ex = (xi + ox) - XY[j].x, ey = (yi + oy) - XY[j].y;
+    const double2 pj = XY[j];
+    const double ex = (xi + ox) - pj.x, ey = (yi + oy) - pj.y;
     const double d2 = ex * ex + ey * ey;
     const uint32_t id = f.id[j];
     int m = n;
     while (m > 0 && (rd[m - 1] > d2 || (rd[m - 1] == d2 && ro[m - 1] > id))) {
       rd[m] = rd[m - 1];
       ro[m] = ro[m - 1];
+      if (rx) {
+        rx[m] = rx[m - 1];
+        ry[m] = ry[m - 1];
+      }
       --m;
     }
     rd[m] = d2;
     ro[m] = id;
+    if (rx) {
+      rx[m] = pj.x - ox;
+      ry[m] = pj.y - oy;
+    }
+  }
+  for (int n = k; n < kout; ++n) {
+    ro[n] = 0xFFFFFFFFu;
+    rd[n] = NAN;
+    if (rx) {
+      rx[n] = NAN;
+      ry[n] = NAN;
+    }
   }
 }
 
 int enqueue_read_neighbours(const PassBuffers& b, uint32_t* dev_ids, double* dev_d2,
-                            hipStream_t st) {
+                            double* dev_px, double* dev_py, hipStream_t st) {
   hipLaunchKernelGGL(k_read_nbrs, dim3(b.nitems), dim3(kItem), 0, st, b.items, b.nitems, b.sims,
-                     b.xyp, b.fcur, b.nbr, b.kmax, b.kout, dev_ids, dev_d2);
+                     b.xyp, b.fcur, b.nbr, b.kmax, b.kout, dev_ids, dev_d2, dev_px, dev_py);
   return (int)hipGetLastError();
 }
 

@@ -177,3 +177,33 @@ def serve_jobs(count: int = 1000, seed: int = 5, lo: int = 1024, hi: int = 1 << 
     k = g.choice(np.array([32, 64]), count)
     steps = g.integers(10, 201, count)
     return [(int(a), int(b), int(c)) for a, b, c in zip(n, k, steps)]
+
+
+SERVE_TEMPLATE_N = 1 << 20
+SERVE_TEMPLATES = 16
+
+
+def serve_ics(seed: int = 5):
+    """Initial conditions of served jobs: 16 clustered templates of 2^20
+    particles (the clustered recipe, template seeds seed*1000+t), shuffled
+    once; job i takes a window of n particles of template t_i from a random
+    start and shifts it by a random periodic offset (all drawn from the job's
+    own seed), so every job has its own clustered layout at a few ms of
+    host time.  Returns make(i, n, k) -> state."""
+    base = []
+    for t in range(SERVE_TEMPLATES):
+        st = clustered_pcg(SERVE_TEMPLATE_N, seed=seed * 1000 + t)
+        perm = np.random.default_rng(seed * 1000 + t).permutation(SERVE_TEMPLATE_N)
+        base.append({c: st[c][perm] for c in ("x", "y", "vx", "vy")})
+
+    def make(i, n, k):
+        g = np.random.default_rng(seed * 10**6 + i)
+        b = base[int(g.integers(SERVE_TEMPLATES))]
+        idx = (int(g.integers(SERVE_TEMPLATE_N)) + np.arange(n)) % SERVE_TEMPLATE_N
+        sx, sy = g.random(2)
+        x = wrap_unit(b["x"][idx] + sx)
+        y = wrap_unit(b["y"][idx] + sy)
+        return {"x": x, "y": y, "vx": b["vx"][idx], "vy": b["vy"][idx], "e": np.ones(n),
+                "mass": 1.0 / n, "k": k, "n": n}
+
+    return make

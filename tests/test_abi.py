@@ -46,7 +46,10 @@ def _create(**kw):
     (dict(kernel=7), "unknown kernel"),
     (dict(precision=9), "unknown precision"),
     (dict(precision=L.PREC_FP32C, flags=L.FLAG_BITWISE), "bitwise"),
-    (dict(hor=(-L.DBL_MAX, L.DBL_MAX)), "open boundaries"),
+    (dict(hor=(-L.DBL_MAX, 1.0)), "open and periodic"),
+    (dict(vert=(0.0, L.DBL_MAX)), "open and periodic"),
+    (dict(n=20, k=32, hor=(-L.DBL_MAX, L.DBL_MAX), vert=(-L.DBL_MAX, L.DBL_MAX)),
+     "fewer than k"),
     (dict(hor=(1, 0)), "positive size"),
     (dict(n=1), "at least 2"),
     (dict(n=3, k=32), "fewer than k"),
@@ -99,6 +102,26 @@ def test_checkpoint_load_rejects_bad_images():
     hdr = struct.pack("<8sIIqqiIIIdQ", b"SPHCKPT1", 1, 64, -5, 0, 0,
                       ctypes.sizeof(L.Params), 0, 14, 4.0, 0)
     assert load(hdr + bytes(64))[0] == -1
+    # valid header and parameters, but a simulation record that is not the
+    # one the parameters give: refused before any device call
+    def header(simdev_bytes):
+        return struct.pack("<8sIIqqiIIIdQ", b"SPHCKPT1", 1, 64, 100, 3, 1,
+                           ctypes.sizeof(L.Params), simdev_bytes, 14, 4.0, 0)
+
+    p = L.default_params(hor=(0, 1), vert=(0, 1), mass=0.01)
+    payload = bytes(100 * (14 * 8 + 4))
+    # the record size of this build: the one size the layout check accepts
+    sizes = [b for b in range(8, 1024, 8)
+             if "layout" not in load(header(b) + bytes(p) + bytes(b) + payload)[1]]
+    assert len(sizes) == 1, sizes
+    simdev_bytes = sizes[0]
+    hdr = header(simdev_bytes)
+    body = bytes(p) + bytes(simdev_bytes) + payload
+    rc, msg = load(hdr + body)
+    assert rc == -1 and "simulation record" in msg, msg
+    bad = L.default_params(hor=(0, 1), vert=(0, 1), mass=0.01, k=99)
+    rc, msg = load(hdr + bytes(bad) + bytes(simdev_bytes) + payload)
+    assert rc == -1 and "parameters" in msg, msg
     assert lib.sph_checkpoint_size(None, None) == -1
     assert lib.sph_checkpoint_save(None, None, 0) == -1
     assert h.value is None

@@ -165,6 +165,32 @@ def test_fp32c_pair_closer_than_fixed_point_step():
         assert np.isfinite(g.get(f)).all(), f
 
 
+def test_fp32c_compact_cluster():
+    # A cluster so compact that h spans only a few thousand steps of the
+    # 32-bit fixed-point positions (h < 2^-11 L): those queries take their
+    # displacements from the FP64 positions, so the density stays within
+    # 1e-5 of the oracle's however small the cluster.
+    rng = np.random.default_rng(12)
+    n_c, n_b = 3000, 1000
+    x = np.concatenate([0.4 + 1e-4 * rng.standard_normal(n_c), rng.random(n_b)])
+    y = np.concatenate([0.6 + 1e-4 * rng.standard_normal(n_c), rng.random(n_b)])
+    n = n_c + n_b
+    st = {"x": x % 1.0, "y": y % 1.0, "vx": 0.01 * rng.standard_normal(n),
+          "vy": 0.01 * rng.standard_normal(n), "e": np.ones(n), "mass": 1.0 / n}
+    g, o = _pair(st, 32, precision=L.PREC_FP32C)
+    h = o.get("h")
+    assert (h < 2.0**-11).sum() > n_c // 2 and (h > 2.0**-11).any()
+    for _ in range(2):
+        _check_knn(g, o)
+        rel = np.abs(g.get("rho") - o.get("rho")) / o.get("rho")
+        assert rel.max() <= 1e-5
+        for f in ("ax", "ay"):
+            a, b = g.get(f), o.get(f)
+            assert np.abs(a - b).max() <= 1e-4 * np.abs(b).max(), f
+        g.calculate_forces()
+        o.calculate_forces()
+
+
 def test_batch_with_different_boxes():
     boxes = [((0, 1), (0, 1)), ((-1, 2), (0, 0.5)), ((10, 11), (10, 13))]
     sts, ps = [], []

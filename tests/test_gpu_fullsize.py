@@ -4,7 +4,16 @@ blast 262,144 (k = 32): the oracle runs a pass in about a second, so the HIP
 path is held to it directly.  clustered 4,194,304 (k = 64): one oracle pass
 (about 20-40 s on the host cores), plus size-independent properties: the
 bitwise form is invariant under relabelling the particles, and the FP32C
-and FP64 forms select identical neighbour sets."""
+and FP64 forms select identical neighbour sets.
+
+The first pass of a simulation searches every query exactly; every later
+pass searches a bracket around each particle's h of the pass before (the
+tile walk, with the exact search only for the queries it cannot resolve).
+The *_walk tests step the simulation, copy its state into the oracle and
+hold one more pass of both to the same bars: neighbour IDs and h bit-exact,
+density and sound speed 1e-12 relative per particle, acceleration and EDot
+1e-12 of the size of their terms per particle (FP64); FP32C: IDs and h
+bit-exact, density 1e-5 relative."""
 import os
 
 import numpy as np
@@ -14,7 +23,7 @@ import oracle as O
 from sphugo_amd import _lib as L
 from sphugo_amd import ics
 
-from test_gpu_parity import _check_forces
+from test_gpu_parity import _check_forces, state_pass_vs_oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -58,7 +67,8 @@ def blast_oracle(blast):
     o = _oracle(blast, 32)
     _first_pass(o, blast)
     ids, d2 = o.neighbours()
-    return {"ids": ids, "d2": d2, **{f: o.get(f) for f in ("h", "rho", "c", "ax", "ay", "edot")}}
+    return {"ids": ids, "d2": d2, **{f: o.get(f) for f in ("h", "rho", "c", "ax", "ay", "edot")},
+            "scale": {f: o.term_scale(f) for f in ("ax", "ay", "edot")}}
 
 
 class _Frozen:
@@ -72,6 +82,9 @@ class _Frozen:
 
     def get(self, f):
         return self.d[f]
+
+    def term_scale(self, f):
+        return self.d["scale"][f]
 
 
 def test_blast_pass_fp64(blast, blast_oracle):
@@ -120,6 +133,7 @@ def test_clustered_pass_fp64_vs_oracle(clustered):
     oh, orho = o.get("h"), o.get("rho")
     oi, _ = o.neighbours()
     oa = {f: o.get(f) for f in ("c", "ax", "ay", "edot")}
+    oa.update({"s" + f: o.term_scale(f) for f in ("ax", "ay", "edot")})
     del o
     g = _gpu(clustered, 64)
     _first_pass(g, clustered)
@@ -132,6 +146,7 @@ def test_clustered_pass_fp64_vs_oracle(clustered):
     for f in ("ax", "ay", "edot"):
         a, b = g.get(f), oa[f]
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), f
+        assert (np.abs(a - b) <= 1e-12 * oa["s" + f]).all(), f
 
 
 def test_clustered_relabel_invariance_bitwise(clustered):
@@ -172,3 +187,41 @@ def test_clustered_forms_agree(clustered):
         e.append(s.reductions()[2])
         s.close()
     assert abs(e[1] - e[0]) <= 1e-8 * abs(e[0])
+
+
+def _walk_vs_oracle(st, k, precision, steps=3):
+    """Step the HIP path, then one pass of it and of the oracle from its
+    copied state (the bracketed walk on the GPU)."""
+    g = _gpu(st, k, precision=precision)
+    g.step(steps)
+    o, nfb = state_pass_vs_oracle(g, O.default_params(knn_mode=O.KNN_EXACT, use_cfl=1,
+                                                      **_params(st, k)), THREADS)
+    # the walk resolved the queries (only a few per cent take the exact search)
+    assert nfb < 0.1 * st["n"], nfb
+    oi, od = o.neighbours()
+    gi, gd = g.neighbours()
+    assert (gi.astype(np.int64) == oi).all()
+    assert (gd == od).all()
+    del gi, gd, oi, od
+    assert (g.get("h") == o.get("h")).all()
+    if precision == L.PREC_FP64:
+        _check_forces(g, o)
+    else:
+        rho, orho = g.get("rho"), o.get("rho")
+        assert (np.abs(rho - orho) / orho).max() <= 1e-5
+
+
+def test_blast_walk_fp64(blast):
+    _walk_vs_oracle(blast, 32, L.PREC_FP64)
+
+
+def test_blast_walk_fp32c(blast):
+    _walk_vs_oracle(blast, 32, L.PREC_FP32C)
+
+
+def test_clustered_walk_fp64(clustered):
+    _walk_vs_oracle(clustered, 64, L.PREC_FP64)
+
+
+def test_clustered_walk_fp32c(clustered):
+    _walk_vs_oracle(clustered, 64, L.PREC_FP32C)

@@ -6,9 +6,10 @@ Bars (stated per check):
   h                     bit-exact (sqrt of the k-th squared distance)
   density, sound speed  FP64: relative 1e-12 (bitwise mode: exact);
                         FP32C: relative 1e-5
-  acceleration, EDot    FP64: |da| <= 1e-12 * max|a| over the set and per
-                        particle 1e-12 * sum_j |term_j| is met through the
-                        normwise bound below; bitwise mode: exact
+  acceleration, EDot    FP64: per particle |da_i| <= 1e-12 * (sum_j
+                        |term_ij| + |g|), the size of what the neighbour sum
+                        cancels (the oracle reports sum_j |term_ij|), and
+                        normwise |da| <= 1e-12 * max|a|; bitwise mode: exact
 """
 import numpy as np
 import pytest
@@ -56,12 +57,36 @@ def _rel(a, b):
     return np.abs(a - b) / np.abs(b)
 
 
-def _check_forces(g, o, tol=1e-12):
+def _check_forces(g, o, tol=1e-12, accel=(0.0, 0.0)):
+    """Density and sound speed per particle; acceleration and EDot per
+    particle against the size of their terms, and normwise."""
     for f in ("rho", "c"):
         assert _rel(g.get(f), o.get(f)).max() <= tol, f
+    gabs = {"ax": abs(accel[0]), "ay": abs(accel[1]), "edot": 0.0}
     for f in ("ax", "ay", "edot"):
         a, b = g.get(f), o.get(f)
         assert np.abs(a - b).max() <= tol * np.abs(b).max(), f
+        scale = o.term_scale(f) + gabs[f]
+        bad = np.abs(a - b) > tol * scale
+        assert not bad.any(), (f, int(bad.sum()), float((np.abs(a - b) / scale).max()))
+
+
+def state_pass_vs_oracle(g, oparams, threads=1):
+    """Copy the GPU state after its steps (positions at rest, velocities,
+    energies, predicted velocities and energies) into an oracle, and run one
+    force pass on both from identical inputs.  The GPU pass searches with
+    the bracket around each particle's h of its last pass (the path every
+    step after step 0 takes).  Returns the oracle and the number of GPU
+    queries that left that path for the exact search."""
+    fields = {f: g.get(f) for f in ("x", "y", "vx", "vy", "e", "vpx", "vpy", "epred")}
+    o = O.OracleSim(oparams, fields["x"], fields["y"], fields["vx"], fields["vy"], fields["e"],
+                    threads=threads)
+    for f in ("vpx", "vpy", "epred"):
+        o.set(f, fields[f])
+    o.calculate_forces()
+    r0 = g.knn_counters()["retries"]
+    g.calculate_forces()
+    return o, g.knn_counters()["retries"] - r0
 
 
 @pytest.mark.parametrize("k", [32, 64])
@@ -189,9 +214,45 @@ def test_invalid_inputs_raise():
     st = ics.uniform(100)
     with pytest.raises(L.SphError):
         L.Sim(L.default_params(k=65, hor=(0, 1), vert=(0, 1)), st["x"], st["y"])
-    with pytest.raises(L.SphError):
-        L.Sim(L.default_params(k=32), st["x"], st["y"])  # open boundaries
+    with pytest.raises(L.SphError):  # open and periodic on one axis
+        L.Sim(L.default_params(k=32, hor=(-L.DBL_MAX, 1.0)), st["x"], st["y"])
     bad = st["x"].copy()
     bad[3] = np.nan
     with pytest.raises(L.SphError):
         L.Sim(L.default_params(k=32, hor=(0, 1), vert=(0, 1)), bad, st["y"])
+
+
+@pytest.mark.parametrize("precision", [L.PREC_FP64, L.PREC_FP32C])
+def test_walk_pass_after_steps(precision):
+    """After three steps every query with an h of its last pass takes the
+    bracketed walk; one more pass from the copied state against the oracle."""
+    st = ics.clustered(20000, seed=31, k=64)
+    common, gpu_only = _params(st, 64, precision, dt_half=0.0, cfl=0.3)
+    g = L.Sim(L.default_params(**common, **gpu_only), st["x"], st["y"], st["vx"], st["vy"],
+              st["e"])
+    g.step(3)
+    o, nfb = state_pass_vs_oracle(g, O.default_params(knn_mode=O.KNN_EXACT, use_cfl=1, **common))
+    assert nfb < 0.1 * st["n"], nfb
+    _check_knn(g, o)
+    if precision == L.PREC_FP64:
+        _check_forces(g, o)
+    else:
+        assert _rel(g.get("rho"), o.get("rho")).max() <= 1e-5
+
+
+def test_neighbour_image_positions():
+    """NNPos (p_j - o through the query's periodic image), as the Animator
+    reads it: bit-exact against the oracle's, row for row; the farthest two
+    (NNPos[0], NNPos[1] of the reference) are row entries k-1 and k-2."""
+    st = ics.uniform(2200, seed=17)
+    g, o = _pair(st, 32)
+    gx, gy = g.neighbour_pos()
+    ox, oy = o.neighbour_pos()
+    assert (gx == ox).all() and (gy == oy).all()
+    # images: some neighbour positions lie outside the unit box
+    assert ((gx < 0) | (gx >= 1)).any()
+    # the image position is consistent with the squared distance
+    _, gd = g.neighbours()
+    x, y = g.get("x"), g.get("y")
+    d2 = (gx - x[:, None]) ** 2 + (gy - y[:, None]) ** 2
+    assert np.allclose(d2, gd, rtol=1e-12, atol=0)
