@@ -157,6 +157,16 @@ int sph_read_neighbours(sph_sim* s, uint32_t* ids, double* d2);
  * (so the reference's NNPos[0], the farthest, is row entry k-1 and NNPos[1]
  * entry k-2; AnimatorGL.AddFrame reads those two, sim/animator.go:171-172). */
 int sph_read_neighbour_pos(sph_sim* s, double* px, double* py);
+/* The same rows as sph_read_neighbours, staged through a fixed device
+ * buffer instead of buffers allocated on the first call: no device memory
+ * is allocated or freed.  No counterpart in the reference (whose neighbour
+ * arrays live in the Go particle slice); for callers that cannot allocate
+ * device memory (stream capture, serving loops, harnesses that forbid it).
+ * Limit: particles x k_max <= 2^20 entries (e.g. 16,384 particles at
+ * k = 64); beyond it SPH_EINVAL.  The buffer (12 MiB per GPU, held until the
+ * process ends) is shared by every call on a GPU: calls that overlap on one
+ * GPU race. */
+int sph_read_neighbours_noalloc(sph_sim* s, uint32_t* ids, double* d2);
 
 /* sum e, sum rho, E = sum m(e + v^2/2), Px, Py, last dt_half (FP64, fixed
  * reduction order). */

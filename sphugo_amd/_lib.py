@@ -69,6 +69,7 @@ SIGNATURES = {
     "sph_write_field": (ctypes.c_int, [_VP, ctypes.c_int32, _D]),
     "sph_read_neighbours": (ctypes.c_int, [_VP, _U32, _D]),
     "sph_read_neighbour_pos": (ctypes.c_int, [_VP, _D, _D]),
+    "sph_read_neighbours_noalloc": (ctypes.c_int, [_VP, _U32, _D]),
     "sph_reductions": (ctypes.c_int, [_VP, _D]),
     "sph_total_energy": (ctypes.c_int, [_VP, _D]),
     "sph_total_density": (ctypes.c_int, [_VP, _D]),

@@ -19,7 +19,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(HERE, "libsphugo_amd.so")
-SOURCES = ["sph_sort.hip", "sph_step.hip", "sph_api.hip", "sph_server.hip"]
+SOURCES = ["sph_sort.hip", "sph_step.hip", "sph_api.hip", "sph_server.hip", "sph_noalloc.hip"]
 BUILD_DIR = os.path.join(HERE, "_build")
 
 FLAGS = [

@@ -242,6 +242,8 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
     delete h;
     return code;
   };
+  if (hipGetDevice(&h->device) != hipSuccess)
+    return bail(fail(SPH_EDEVICE, "hipGetDevice failed"));
   if (hipStreamCreateWithFlags(&h->st, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "hipStreamCreate failed"));
   if (h->timing)
