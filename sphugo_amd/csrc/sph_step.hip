@@ -737,15 +737,25 @@ __device__ inline void write_result(const SimDev* S, int64_t t, double xq, doubl
 
 // FP32C: may the displacements of a query with support h come from the 32-bit
 // fixed-point positions (units of L / 2^32)?  Their 32-bit difference is the
-// minimum image while every neighbour lies within half a box (h < 0.45 L),
-// and its quantisation (< one step per component) stays below 2^-21 h while
-// h > 2^-11 L: q moves by under 7e-7, so a 64-term density sum of F
-// (|F'| <= 1/3) moves by under 1.5e-5 even with every error aligned, 8e-6 of
-// the sum of a uniform neighbourhood (about 1.9).  Otherwise the FP64
-// positions give the displacement.
+// minimum image while every neighbour lies within half a box (h < 0.45 L).
+// The displacement is centred on the query's exact position (centred_step):
+// what is left is the neighbour's flooring, uniform within half a step and
+// independent between neighbours, below 2^-20 h per component while
+// h > 2^-13 L.  Emulated on clusters reaching that threshold the density
+// stays within 2e-6 of FP64.  Otherwise the FP64 positions give the
+// displacement.
 __device__ inline bool fixed_point_ok(double h, double lx, double ly) {
   const double lmin = lx < ly ? lx : ly;
-  return h < 0.45 * lmin && h > 0x1p-11 * lmin;
+  return h < 0.45 * lmin && h > 0x1p-13 * lmin;
+}
+
+// Offset, in fixed-point steps, from a particle's exact position to the
+// centre of its fixed-point step (1/2 minus the fraction the floor dropped):
+// the displacement to a neighbour j is (u_j - u_i) + this, whose error is
+// j's flooring alone, centred on zero.
+__device__ inline float centred_step(double x, double x0, double l) {
+  const double f = (x - x0) * (4294967296.0 / l);
+  return (float)(0.5 - (f - floor(f)));
 }
 
 // First-pass search radius: local density from the occupancy of the
@@ -1364,10 +1374,11 @@ __global__ __launch_bounds__(256) void k_knn_finish(
       if (fixed_point_ok(h, lx, ly)) {
         const uint2 me = UXY[t];
         const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
+        const float cx = centred_step(xq, S->x0, lx) * gx, cy = centred_step(yq, S->y0, ly) * gy;
         row_visit<8>(row, k, (uint32_t)t, uxys, [&](uint32_t, uint2 pj) {
 #pragma clang fp contract(fast)
-          const float ex = (float)(int32_t)(pj.x - me.x) * gx;
-          const float ey = (float)(int32_t)(pj.y - me.y) * gy;
+          const float ex = __builtin_fmaf((float)(int32_t)(pj.x - me.x), gx, cx);
+          const float ey = __builtin_fmaf((float)(int32_t)(pj.y - me.y), gy, cy);
           kahan(__builtin_amdgcn_sqrtf(__builtin_fmaf(ey, ey, ex * ex)));
         });
       } else {
@@ -1559,6 +1570,7 @@ __global__ __launch_bounds__(256) void k_force(
     // of L / 2^32; otherwise from FP64
     const bool fixed = fixed_point_ok(hi, lx, ly);
     const float fsx = (float)(lx * 0x1p-32), fsy = (float)(ly * 0x1p-32);
+    const float csx = centred_step(xi, S->x0, lx) * fsx, csy = centred_step(yi, S->y0, ly) * fsy;
     ci = 2.0 * (double)me.c;
     rhoi = 2.0 * (double)me.rho;
     A = (double)me.B;
@@ -1571,10 +1583,11 @@ __global__ __launch_bounds__(256) void k_force(
     auto pair = [&](float rx, float ry, const RecF& nb) {
 #pragma clang fp contract(fast)
       const float d2 = rx * rx + ry * ry;
-      // Two particles closer than the fixed-point step (L / 2^32) have a
-      // zero displacement here; their pair term vanishes with the distance
-      // (DF(q) ~ q and dot ~ r as r -> 0): the reciprocal square root of at
-      // least FLT_MIN keeps it 0 (q = 0, DF(0) = 0, r = 0) rather than the
+      // Two particles within one fixed-point step (L / 2^32) can get a zero
+      // displacement here (or one from the FP64 positions of coincident
+      // particles); their pair term vanishes with the distance (DF(q) ~ q
+      // and dot ~ r as r -> 0): the reciprocal square root of at least
+      // FLT_MIN keeps it 0 (q = 0, DF(0) = 0, r = 0) rather than the
       // 0 * inf of rsq(0).
       const float dinv = __builtin_amdgcn_rsqf(__builtin_fmaxf(d2, 0x1p-126f));
       const float q = __builtin_fminf((d2 * dinv) * hinv, 1.f);
@@ -1611,14 +1624,15 @@ __global__ __launch_bounds__(256) void k_force(
     // runs a loop without the FP64 form, so every record loads whole.
     if (__all(fixed)) {
       row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t, const RecF& nb) {
-        pair((float)(int32_t)(nb.ux - me.ux) * fsx, (float)(int32_t)(nb.uy - me.uy) * fsy, nb);
+        pair(__builtin_fmaf((float)(int32_t)(nb.ux - me.ux), fsx, csx),
+             __builtin_fmaf((float)(int32_t)(nb.uy - me.uy), fsy, csy), nb);
       });
     } else {
       row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t pk, const RecF& nb) {
         float rx, ry;
         if (fixed) {
-          rx = (float)(int32_t)(nb.ux - me.ux) * fsx;
-          ry = (float)(int32_t)(nb.uy - me.uy) * fsy;
+          rx = __builtin_fmaf((float)(int32_t)(nb.ux - me.ux), fsx, csx);
+          ry = __builtin_fmaf((float)(int32_t)(nb.uy - me.uy), fsy, csy);
         } else {
           const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
           const double2 pj = XY[pk >> 4];

@@ -166,10 +166,11 @@ def test_fp32c_pair_closer_than_fixed_point_step():
 
 
 def test_fp32c_compact_cluster():
-    # A cluster so compact that h spans only a few thousand steps of the
-    # 32-bit fixed-point positions (h < 2^-11 L): those queries take their
+    # A cluster so compact that h spans only a few hundred steps of the
+    # 32-bit fixed-point positions (h < 2^-13 L): those queries take their
     # displacements from the FP64 positions, so the density stays within
-    # 1e-5 of the oracle's however small the cluster.
+    # 1e-5 of the oracle's however small the cluster.  Its background
+    # neighbours see the whole cluster near q = 1.
     rng = np.random.default_rng(12)
     n_c, n_b = 3000, 1000
     x = np.concatenate([0.4 + 1e-4 * rng.standard_normal(n_c), rng.random(n_b)])
@@ -179,7 +180,7 @@ def test_fp32c_compact_cluster():
           "vy": 0.01 * rng.standard_normal(n), "e": np.ones(n), "mass": 1.0 / n}
     g, o = _pair(st, 32, precision=L.PREC_FP32C)
     h = o.get("h")
-    assert (h < 2.0**-11).sum() > n_c // 2 and (h > 2.0**-11).any()
+    assert (h < 2.0**-13).sum() > n_c // 2 and (h > 2.0**-13).any()
     for _ in range(2):
         _check_knn(g, o)
         rel = np.abs(g.get("rho") - o.get("rho")) / o.get("rho")
