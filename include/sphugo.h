@@ -235,7 +235,7 @@ int sph_batch_reductions(sph_sim* s, int32_t sim, double out[6]);
 /* Replay regular steps from a captured hipGraph (1) or launch them one by
  * one (0).  Results are identical either way. */
 int sph_set_graph(sph_sim* s, int32_t on);
-/* Time force passes over the candidate Morton grid resolutions (cells per
+/* Time force passes over the candidate cell grid resolutions (cells per
  * particle 0.5 ... 16) on the current state and keep the fastest.  Only
  * before the first step (SPH_ESTATE otherwise).  Either output may be NULL. */
 int sph_autotune(sph_sim* s, double* best_ms, double* best_cells_per_particle);

@@ -82,7 +82,7 @@ int sph::check_params(const sph_params* p, int64_t n) {
 }
 
 // log2 of cells per axis: a power of two with about `cpp` cells per
-// particle at the mean density (Morton-ordered cells; coarser levels of the
+// particle at the mean density (Hilbert-ordered cells; coarser levels of the
 // same grid serve the denser regions' neighbours and the sparser regions'
 // search boxes).
 int sph::grid_log2(int64_t n, double cpp) {

@@ -4,7 +4,7 @@
 // The pool holds `capacity` particle slots.  A job occupies a contiguous
 // range [off, off + n) and a simulation slot (its SimDev); unused ranges are
 // holes.  Every tick runs ONE force pass over the whole pool: the radix sort
-// keys a job's particles by coff + Morton(cell) with coff = off * 16 + 1 and a
+// keys a job's particles by coff + Hilbert(cell) with coff = off * 16 + 1 and a
 // hole slot t by t * 16, so the sort keeps every job inside its own range and
 // every hole in place, and each job's cells never collide with another's.
 // Each job carries its own pass modes (first pass: initial kNN + forces
@@ -16,7 +16,7 @@
 //
 // Admission is FIFO: the head of the queue waits until a slot and a
 // first-fit range of n slots are free.  One native worker thread owns the
-// device; clients submit and wait from any thread.  The Morton grid of a job
+// device; clients submit and wait from any thread.  The cell grid of a job
 // is the one a standalone handle uses (about 4 cells per particle), so a job
 // produces exactly the results of sph_create + sph_step with the same
 // parameters, whatever else shares the pool.

@@ -62,7 +62,7 @@ struct sph_sim {
   // graph replay of a regular step, one per field-buffer parity
   bool graph_on = false;
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
-  double grid_cpp = 4.0;        // cells per particle of the Morton grid
+  double grid_cpp = 4.0;        // cells per particle of the cell grid
   int64_t cell_capacity = 0;    // cstart entries allocated
 
   template <class T>

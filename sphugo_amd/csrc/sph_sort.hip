@@ -1,7 +1,7 @@
 // Stable LSD radix sort of (cell key, particle index) pairs, 8-bit digits.
 //
 // Replaces the reference's per-pass Treebuild/Partition permutation
-// (sim/core.go:126-224): particles are reordered by the Morton (Z-order)
+// (sim/core.go:126-224): particles are reordered by the Hilbert-curve
 // index of their cell in a uniform 2^lg x 2^lg grid, so that every aligned
 // square block of cells, at every level, is one contiguous range of the
 // state arrays.

@@ -1,6 +1,6 @@
 // SPH force pass and integrator kernels for gfx950 (wave64).
 //
-//   k_keys          cell key of each particle's pass position: Morton code
+//   k_keys          cell key of each particle's pass position: Hilbert index
 //                   of a power-of-two grid plus the simulation's cell offset
 //                   (replaces Treebuild/Partition, sim/core.go:126-224)
 //   sort_pairs      stable LSD radix sort of (key, index) (sph_sort.hip)
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
   }
   const int64_t cx = c.ux - (floordiv_pow2(c.ux, S->lg) << S->lg);
   const int64_t cy = c.uy - (floordiv_pow2(c.uy, S->lg) << S->lg);
-  keys[t] = (uint32_t)(S->coff + morton2((uint32_t)cx, (uint32_t)cy));
+  keys[t] = (uint32_t)(S->coff + hilbert2((uint32_t)cx, (uint32_t)cy, S->lg));
 }
 
 __global__ __launch_bounds__(kItem) void k_permute(
@@ -298,14 +298,14 @@ __global__ __launch_bounds__(kItem) void k_permute(
 // ------------------------------------------------------------------------
 // kNN.
 //
-// Candidates are the particles of Morton cells at a coarser level of the
+// Candidates are the particles of the cells at a coarser level of the
 // grid: a coarse cell (X, Y) at level c is one contiguous range of the
 // sorted arrays, [cstart[m << 2(lg-c)], cstart[(m+1) << 2(lg-c)]) with m
-// its Morton code.  Periodic images follow the reference: a candidate j is
-// seen through the query offsets o in {-L, 0, L}^2 and its squared distance
-// is ((p_i + o) - p_j)^2 (sim/nearest-neighbour.go:57-61,72,76), self
-// excluded at every image (:79).  The k nearest are ordered by (d2, ID,
-// image) so ties resolve the same way everywhere.
+// its Hilbert index at that level.  Periodic images follow the reference:
+// a candidate j is seen through the query offsets o in {-L, 0, L}^2 and its
+// squared distance is ((p_i + o) - p_j)^2 (sim/nearest-neighbour.go:57-61,
+// 72,76), self excluded at every image (:79).  The k nearest are ordered by
+// (d2, ID, image) so ties resolve the same way everywhere.
 
 __device__ inline int level_for(const SimDev* S, double w, double hgt, double ncells) {
   int c = S->lg;
@@ -317,7 +317,7 @@ __device__ inline int level_for(const SimDev* S, double w, double hgt, double nc
 __device__ inline void cell_range(const SimDev* S, const uint32_t* __restrict__ cstart, int c,
                                   uint32_t xw, uint32_t yw, uint32_t& a, uint32_t& b) {
   const int sh = 2 * (S->lg - c);
-  const uint64_t m = (uint64_t)morton2(xw, yw) << sh;
+  const uint64_t m = (uint64_t)hilbert2(xw, yw, c) << sh;
   a = cstart[S->coff + m];
   b = cstart[S->coff + m + (1ull << sh)];
 }
@@ -780,7 +780,7 @@ __device__ inline double local_radius(const SimDev* S, const uint32_t* __restric
 }
 
 // kNN + density + sound speed + force record.  A wave takes 64 consecutive
-// (Morton-ordered) queries.  Fast path (lane = query): the wave stages the
+// (Hilbert-ordered) queries.  Fast path (lane = query): the wave stages the
 // candidates of the tile's bounding box (grown by each query's bracket
 // radius) through LDS 64 at a time and every lane tests all of them against
 // its bracket [h_prev (1 - delta), h_prev (1 + delta)]: nearer candidates
@@ -839,7 +839,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
     // index fill the same lines
     uint32_t* annrow = ann + (t >> 6) * (int64_t)(64 * kAnn) + (t & 63);
     // Lane groups: aligned blocks of consecutive queries.  A block whose
-    // box holds more than kSplit candidates per query is halved (Morton
+    // box holds more than kSplit candidates per query is halved (cell
     // order can put two distant regions in one wave); blocks below 8 lanes
     // go to the wavefront search, which handles them at lower cost.
     unsigned long long pending = ballot(fast);
@@ -1173,7 +1173,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
 // `self`.  NT: the row is read for the last time (non-temporal loads).
 // A source array read through a block's LDS window: entries [wb, wb + W)
 // (indices mod 2^32) come from the staged copy, the rest from the array.
-// Particles are in Morton order, so most neighbours of a block's 256
+// Particles are in Hilbert order, so most neighbours of a block's 256
 // consecutive queries lie within a few hundred slots of them.
 template <typename T, int W>
 struct WinSrc {
@@ -1247,7 +1247,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
   const int64_t t = it.first + (valid ? threadIdx.x : 0);
   const uint32_t cp = cnt[t];
   // the block's window of positions in LDS for the density sums, slots
-  // [first - 128, first + 384) (most neighbours of Morton-ordered queries)
+  // [first - 128, first + 384) (most neighbours of Hilbert-ordered queries)
   constexpr int kWin = 512;
   constexpr int kBack = 128;
   __shared__ double2 wxy[kWin];
@@ -1490,7 +1490,7 @@ __global__ __launch_bounds__(256) void k_force(
   if (mode < 0) mode = S->fmode;
   // the block's window of neighbour records in LDS: slots [first - 128,
   // first + 384) (16 KB FP32C, 32 KB FP64), which hold about 85 % of the
-  // neighbours of Morton-ordered particles; the rest are gathered from
+  // neighbours of Hilbert-ordered particles; the rest are gathered from
   // memory.  Force time at 4M clustered, k = 64: FP64 1.17 -> 0.87 ms,
   // FP32C 0.57 -> 0.49 ms (with 4 records in flight instead of 8).
   constexpr int kWin = 512;

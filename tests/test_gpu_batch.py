@@ -2,7 +2,7 @@
 
 Bars: a simulation stepped inside a batch is bit-identical to the same
 simulation stepped alone; graph replays are bit-identical to launching the
-kernels one by one; the autotuner changes the Morton grid but not the
+kernels one by one; the autotuner changes the cell grid but not the
 neighbour sets, and in the bitwise form not a single result bit.
 """
 import numpy as np
