@@ -352,6 +352,7 @@ struct TileShared {
   uint32_t spk[65];
   double sbb[kSub][4];       // box of each sub-tile (x0, x1, y0, y1)
   uint8_t slist[kSub][64];   // staged candidates within reach of each sub-tile
+  uint32_t slpk[kSub][64];   // their packed entries, in the same order
   unsigned long long smask[kSub];  // staged slots in each sub-tile's list
   unsigned long long selfm[64];    // per query: staged slots that are itself
 };
@@ -789,11 +790,11 @@ __device__ inline double local_radius(const SimDev* S, const uint32_t* __restric
 // wave's LDS small) from which the remaining ones are selected exactly.
 // Queries without a previous h, or whose bracket misses, fall back to the
 // exact wavefront search.  BITWISE uses the wavefront search for all.
-// Six waves per SIMD: the walk is issue-bound with long LDS and gather
-// latencies, and the extra waves (at the cost of a few spilled registers)
-// hide more of them than the registers they take away.
+// Five waves per SIMD (a few spilled registers): the walk is issue-bound
+// with long LDS and gather latencies; its LDS (about 7 KB per wave) allows
+// five blocks per CU.
 template <bool BITWISE, bool FP32C>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k_knn(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_knn(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const double2* __restrict__ XY,
     const uint8_t* __restrict__ WR, const uint32_t* __restrict__ cstart,
@@ -999,7 +1000,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           for (int g = 0; g < kSub; ++g) {
             const bool in = px >= sbx0[g] && px <= sbx1[g] && py >= sby0[g] && py <= sby1[g];
             const unsigned long long m = ballot(in);
-            if (in) ts.slist[g][lane_prefix(m)] = (uint8_t)lane;
+            if (in) {
+              const int pos = lane_prefix(m);
+              ts.slist[g][pos] = (uint8_t)lane;
+              ts.slpk[g][pos] = cpk;
+            }
             if (lane == 0) ts.smask[g] = m;
             const int c = __popcll(m);
             nm = c > nm ? c : nm;
@@ -1008,6 +1013,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
         __builtin_amdgcn_wave_barrier();
         nm = (__builtin_amdgcn_readfirstlane(nm) + 3) & ~3;
         const uint8_t* mylist = ts.slist[my_sub];
+        const uint32_t* mypk = ts.slpk[my_sub];
         // list positions of this query's own images (every image of the
         // query itself is excluded): a staged slot u in the sub-tile's list
         // sits at the count of list slots below u
@@ -1078,9 +1084,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
           const uint32_t keep = grp ? ~(uint32_t)(spos >> mb) : 0u;
           cmask = (__builtin_bitreverse32(cmask) >> sh) & keep;
           uint32_t amask = (__builtin_bitreverse32(hmask) >> sh) & keep & ~cmask;
-          const uint8_t* blist = mylist + mb;
+          // the entries of the block's positions, read once per emitted bit
+          const uint32_t* blpk = mypk + mb;
           while (cmask) {
-            const uint32_t pk = ts.spk[blist[__builtin_ctz(cmask)]];
+            const uint32_t pk = blpk[__builtin_ctz(cmask)];
             cmask &= cmask - 1u;
             cbuf.x = cbuf.y;
             cbuf.y = cbuf.z;
@@ -1093,7 +1100,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void k
             ++c_lo;
           }
           while (amask) {
-            const uint32_t pk = ts.spk[blist[__builtin_ctz(amask)]];
+            const uint32_t pk = blpk[__builtin_ctz(amask)];
             amask &= amask - 1u;
             if (c_an < kAnn) annrow[64 * c_an] = pk;
             ++c_an;
