@@ -231,6 +231,9 @@ def run_serve(njobs, precision, capacity, max_jobs=256, load=0.7):
         "mean_particle_steps_per_job": round(mean_ps, 1),
         "throughput_particle_steps_per_s": round(rs["particle_steps_per_s"], 1),
         "p50_ms": round(rs["p50_ms"], 2), "p99_ms": round(rs["p99_ms"], 2),
+        "queue_wait_p50_ms": round(rs["queue_p50_ms"], 2),
+        "queue_wait_p99_ms": round(rs["queue_p99_ms"], 2),
+        "run_p50_ms": round(rs["run_p50_ms"], 2), "run_p99_ms": round(rs["run_p99_ms"], 2),
         "mean_latency_ms": round(float(np.mean(lat)), 2),
         "mean_occupancy": round(rs["occupancy"], 4), "ticks": int(rs["ticks"]),
         "tick_ms": round(rs["tick_ms"], 3), "tick_gpu_ms": round(rs["tick_gpu_ms"], 3),
@@ -310,6 +313,8 @@ def main():
     ap.add_argument("--jobs", type=int, default=1000, help="served jobs")
     ap.add_argument("--serve-precision", default="fp32c", choices=["fp32c", "fp64"])
     ap.add_argument("--serve-capacity", type=int, default=1 << 23, help="pool particle slots")
+    ap.add_argument("--serve-only", action="store_true",
+                    help="run only the serving workload and print its object")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
@@ -317,6 +322,10 @@ def main():
         sys.exit(2)
     if args.errors:
         print(json.dumps({"fp32c_error_report": error_report()}), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serve": run_serve(args.jobs, args.serve_precision,
+                                             args.serve_capacity)}), flush=True)
         return
     st = make_state(args)
     k = st["k"]

@@ -285,8 +285,10 @@ int sph_server_release(sph_server* s, int64_t job_id);
  * to the last completion, particle-steps/s over that span, p50 and p99
  * submit-to-done latency (ms), mean pool occupancy, ticks, jobs queued or
  * running now, mean tick duration (ms: uploads, one pass, readbacks) and
- * its device part (ms between events at the tick's start and end). */
-#define SPH_SERVER_NSTATS 12
+ * its device part (ms between events at the tick's start and end), then the
+ * two parts of the latency: p50 and p99 of the queue wait (submit to
+ * admission, ms) and of the run time (admission to done, ms). */
+#define SPH_SERVER_NSTATS 16
 int sph_server_stats(sph_server* s, double out[SPH_SERVER_NSTATS]);
 
 const char* sph_last_error(void);

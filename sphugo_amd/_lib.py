@@ -374,7 +374,8 @@ def autotune(sim):
 
 JOB_STATES = ["queued", "running", "done", "failed"]
 SERVER_STATS = ["jobs_done", "jobs_failed", "particle_steps", "wall_s", "particle_steps_per_s",
-                "p50_ms", "p99_ms", "occupancy", "ticks", "in_flight", "tick_ms", "tick_gpu_ms"]
+                "p50_ms", "p99_ms", "occupancy", "ticks", "in_flight", "tick_ms", "tick_gpu_ms",
+                "queue_p50_ms", "queue_p99_ms", "run_p50_ms", "run_p99_ms"]
 RESULT_FIELDS = ["x", "y", "vx", "vy", "rho", "e", "h"]
 
 

@@ -119,7 +119,9 @@ struct sph_server {
   hipEvent_t ev_tick[2] = {nullptr, nullptr};
   bool have_t0 = false;
   Clock::time_point t0, t_last;
-  std::vector<double> lat_ms;
+  std::vector<double> lat_ms;    // submit to done, per finished job
+  std::vector<double> queue_ms;  // submit to admission
+  std::vector<double> run_ms;    // admission to done
 };
 
 namespace {
@@ -406,6 +408,8 @@ bool tick(sph_server* s) {
       s->n_done += 1;
       s->particle_steps += j->n * (int64_t)j->nsteps;
       s->lat_ms.push_back(ms_between(j->t_submit, now));
+      s->queue_ms.push_back(ms_between(j->t_submit, j->t_start));
+      s->run_ms.push_back(ms_between(j->t_start, now));
     }
     s->t_last = now;
     give_range(s, j->off, j->n);
@@ -656,13 +660,13 @@ extern "C" int sph_server_release(sph_server* s, int64_t job_id) {
 extern "C" int sph_server_stats(sph_server* s, double out[SPH_SERVER_NSTATS]) {
   if (!s || !out) return fail(SPH_EINVAL, "null argument");
   std::lock_guard<std::mutex> lk(s->mu);
-  std::vector<double> l = s->lat_ms;
-  std::sort(l.begin(), l.end());
-  auto pct = [&](double q) {
-    if (l.empty()) return 0.0;
-    size_t i = (size_t)std::ceil(q * l.size());
+  // nearest-rank percentile of a copy of v
+  auto pct = [](std::vector<double> v, double q) {
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    size_t i = (size_t)std::ceil(q * v.size());
     i = i > 0 ? i - 1 : 0;
-    return l[i < l.size() ? i : l.size() - 1];
+    return v[i < v.size() ? i : v.size() - 1];
   };
   const double wall = s->have_t0 && s->n_done ? ms_between(s->t0, s->t_last) / 1e3 : 0.0;
   int64_t queued = (int64_t)s->queue.size(), running = 0;
@@ -672,12 +676,16 @@ extern "C" int sph_server_stats(sph_server* s, double out[SPH_SERVER_NSTATS]) {
   out[2] = (double)s->particle_steps;
   out[3] = wall;
   out[4] = wall > 0 ? s->particle_steps / wall : 0.0;
-  out[5] = pct(0.50);
-  out[6] = pct(0.99);
+  out[5] = pct(s->lat_ms, 0.50);
+  out[6] = pct(s->lat_ms, 0.99);
   out[7] = s->ticks ? s->occupancy_sum / s->ticks : 0.0;
   out[8] = (double)s->ticks;
   out[9] = (double)(queued + running);
   out[10] = s->ticks ? s->tick_ms_sum / s->ticks : 0.0;
   out[11] = s->ticks ? s->gpu_ms_sum / s->ticks : 0.0;
+  out[12] = pct(s->queue_ms, 0.50);
+  out[13] = pct(s->queue_ms, 0.99);
+  out[14] = pct(s->run_ms, 0.50);
+  out[15] = pct(s->run_ms, 0.99);
   return 0;
 }
