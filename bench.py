@@ -93,7 +93,8 @@ def make_sim(st, precision, graph=False, tune=False):
         sim = L.Sim(params_for(st, precision), st["x"], st["y"], st["vx"], st["vy"], st["e"])
     tuned = None
     if tune:
-        tuned = L.autotune(sim)
+        ms, _ = L.autotune(sim)
+        tuned = {"pass_ms": round(ms, 4), **L.get_tuning(sim)}
     if graph:
         L.set_graph(sim, True)
     return sim, tuned
@@ -127,34 +128,79 @@ def run_variant(st, precision, steps, warmup, timing_steps=3, graph=False, tune=
         "knn_retries": retries,
         "knn_counters": counters,
         "dt_half_last": red[5],
-        "autotune": None if tuned is None else {"pass_ms": round(tuned[0], 4),
-                                                "cells_per_particle": tuned[1]},
+        "autotune": tuned,
     }
 
 
-def cpu_baseline(threads_all: int):
+def run_batch(steps=10, warmup=3):
+    """The batched workload: 512 blast simulations, N log-uniform in [1024,
+    65536] (seed 4), k = 32, stepped in one launch per kernel, grid and walk
+    shape autotuned, steps replayed from a hipGraph; FP32C and plain FP64."""
+    args = argparse.Namespace(workload="batch", batch_count=512, n=0)
+    st = make_state(args)
+    out = {"workload": f"batch-512 (blast ICs, N log-uniform in [1024, 65536], seed 4, k=32)",
+           "n_particles": st["n"], "steps": steps, "warmup": warmup, "graph": True}
+    for prec, code in (("fp32c", L.PREC_FP32C), ("fp64", L.PREC_FP64)):
+        r = run_variant(st, code, steps, warmup, graph=True, tune=True)
+        out[prec] = {"value": round(r["value"], 1), "ms_per_step": round(r["ms_per_step"], 4),
+                     "kernel_ms": r["kernel_ms"], "autotune": r["autotune"]}
+        print(f"# batch {prec}: {r}", file=sys.stderr, flush=True)
+    out["speedup_fp32c_over_fp64"] = round(out["fp32c"]["value"] / out["fp64"]["value"], 3)
+    return out
+
+
+def cpu_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": avail, "os_cpu_count": os.cpu_count()}
+
+
+def cpu_baseline(threads_all: int, c3_steps: int = 1):
+    """The oracle's faithful restatement of the reference (Treebuild +
+    BoundingSpheres + 9-image tree kNN + density + force + DKD, gcc -O2
+    -ffp-contract=off) on this host: at the headline size (clustered 4M,
+    k = 64) on all cores (OpenMP over particles in the kNN, density and force
+    loops; the tree build stays serial, as in the reference), and one thread
+    on a 32,768-particle sample of the same layout."""
     import oracle as O
     O.build()
-    st = ics.clustered(32768, seed=3, k=64)
-    out = {}
-    for threads in (1, threads_all):
+
+    def rate(st, threads, nsteps):
         p = O.default_params(k=64, knn_mode=O.KNN_TREE, gamma=5 / 3, mass=st["mass"],
                              hor=(0, 1), vert=(0, 1), use_cfl=1, cfl=0.3,
                              visc_alpha=1.0, visc_beta=2.0)
         s = O.OracleSim(p, st["x"], st["y"], st["vx"], st["vy"], st["e"], threads=threads)
-        s.step(1)  # step 0 pass + first step, untimed
-        nsteps = 4
+        s.step(1)  # step-0 pass + first step, untimed
         t0 = time.perf_counter()
         s.step(nsteps)
         dt = time.perf_counter() - t0
-        out[threads] = st["n"] * nsteps / dt
+        return st["n"] * nsteps / dt, dt / nsteps
+
+    big = ics.clustered(4194304, seed=3, k=64)
+    v_all, s_all = rate(big, threads_all, c3_steps)
+    print(f"# cpu baseline: 4M on {threads_all} threads, {s_all:.1f} s per step",
+          file=sys.stderr, flush=True)
+    small = ics.clustered(32768, seed=3, k=64)
+    v_one, s_one = rate(small, 1, 4)
     return {
-        "value": out[1], "unit": "particle-steps/s", "cores": 1, "kind": "port",
-        "sample": "clustered layout at N=32768, k=64: oracle restatement of the reference "
-                  "(Treebuild + BoundingSpheres + 9-image tree kNN + density + force + DKD), "
-                  "4 steps after step 0, 1 thread, gcc -O2 -ffp-contract=off",
-        "all_cores": {"value": out[threads_all], "cores": threads_all,
-                      "note": "OpenMP over particles in kNN/density/force; tree build serial"},
+        "value": v_all, "unit": "particle-steps/s", "cores": threads_all, "kind": "port",
+        "sample": f"clustered 4,194,304 particles, k=64 (the headline workload): {c3_steps} "
+                  "step(s) timed after the untimed step-0 pass and first step, "
+                  f"{s_all:.1f} s per step; oracle restatement of the reference's tree "
+                  "path, OpenMP over particles in kNN/density/force, tree build serial",
+        **cpu_info(),
+        "one_thread": {"value": v_one, "cores": 1, "seconds_per_step": round(s_one, 3),
+                       "sample": "clustered layout at N=32768, k=64, 4 steps after step 0"},
     }
 
 
@@ -304,10 +350,13 @@ def main():
     ap.add_argument("--n", type=int, default=0)
     ap.add_argument("--precision", default="both", choices=["both", "fp32c", "fp64"])
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=3,
+                    help="timed steps of the all-cores CPU baseline at 4M")
     ap.add_argument("--graph", action="store_true", help="replay steps from a hipGraph")
     ap.add_argument("--tune", action="store_true", help="autotune the grid before stepping")
     ap.add_argument("--batch-count", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--errors", action="store_true",
                     help="print the FP32C error report (FP32C vs FP64) and exit")
     ap.add_argument("--jobs", type=int, default=1000, help="served jobs")
@@ -380,9 +429,11 @@ def main():
         line["fp32c_kernel_ms"] = res["fp32c"]["kernel_ms"]
     if "fp32c" in res and "fp64" in res:
         line["speedup_fp32c_over_fp64"] = round(res["fp32c"]["value"] / res["fp64"]["value"], 3)
+    if not args.no_batch and args.workload == "clustered":
+        line["batch"] = run_batch()
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        line["cpu_baseline"] = cpu_baseline(threads)
+        threads = min(16, cpu_info()["nproc"])
+        line["cpu_baseline"] = cpu_baseline(threads, args.cpu_steps)
     if not args.no_serve and args.workload == "clustered":
         line["serve"] = run_serve(args.jobs, args.serve_precision, args.serve_capacity)
         print(f"# serve: {line['serve']}", file=sys.stderr, flush=True)

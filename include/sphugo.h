@@ -236,9 +236,21 @@ int sph_batch_reductions(sph_sim* s, int32_t sim, double out[6]);
  * one (0).  Results are identical either way. */
 int sph_set_graph(sph_sim* s, int32_t on);
 /* Time force passes over the candidate cell grid resolutions (cells per
- * particle 0.5 ... 16) on the current state and keep the fastest.  Only
- * before the first step (SPH_ESTATE otherwise).  Either output may be NULL. */
+ * particle 0.5 ... 16) on the current state and keep the fastest, then tune
+ * the kNN walk's tile shape on that grid one knob at a time: sub-tiles per
+ * wave (8, 4), the lane group a wave starts with (64, 32 queries), and the
+ * candidates per query above which a group is halved (24, 32, 16).  Only
+ * before the first step (SPH_ESTATE otherwise).  Either output may be NULL.
+ * Outside the bitwise form the grid and shape change the order in which
+ * neighbours are summed (results within the precision form's contract). */
 int sph_autotune(sph_sim* s, double* best_ms, double* best_cells_per_particle);
+/* The grid and walk shape in use: cells per particle, bracket half-width
+ * around each query's last h, split threshold, first lane group, sub-tiles. */
+int sph_get_tuning(sph_sim* s, double out[5]);
+/* Set them (same order and ranges: a grid the autotuner may choose, bracket
+ * 0.005..0.1, split 8..64, group 16/32/64, sub-tiles 4/8); SPH_EINVAL
+ * otherwise.  Any time between steps. */
+int sph_set_tuning(sph_sim* s, const double in[5]);
 
 /* ---- serve loop: continuous batching (no reference counterpart) ----
  * A native worker thread steps every admitted job in one shared device

@@ -96,6 +96,8 @@ SIGNATURES = {
     "sph_checkpoint_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64, ctypes.POINTER(_VP)]),
     "sph_set_graph": (ctypes.c_int, [_VP, ctypes.c_int32]),
     "sph_autotune": (ctypes.c_int, [_VP, _D, _D]),
+    "sph_get_tuning": (ctypes.c_int, [_VP, _D]),
+    "sph_set_tuning": (ctypes.c_int, [_VP, _D]),
     "sph_server_create": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.c_int32,
                                          ctypes.c_int32, ctypes.c_uint32, ctypes.POINTER(_VP)]),
     "sph_server_destroy": (None, [_VP]),
@@ -370,6 +372,22 @@ def autotune(sim):
     cpp = ctypes.c_double()
     check(lib().sph_autotune(sim._h, ctypes.byref(ms), ctypes.byref(cpp)))
     return ms.value, cpp.value
+
+
+TUNING = ["cells_per_particle", "delta", "split", "group", "subtiles"]
+
+
+def get_tuning(sim):
+    out = np.zeros(5)
+    check(lib().sph_get_tuning(sim._h, dptr(out)))
+    return dict(zip(TUNING, out.tolist()))
+
+
+def set_tuning(sim, **kw):
+    cur = get_tuning(sim)
+    cur.update(kw)
+    v = np.array([cur[n] for n in TUNING], np.float64)
+    check(lib().sph_set_tuning(sim._h, dptr(v)))
 
 
 JOB_STATES = ["queued", "running", "done", "failed"]

@@ -292,6 +292,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   pb.prec = h->prec;
   pb.bitwise = h->bitwise;
   pb.np = np;
+  pb.walk = kWalkDefault;
   // device memory
   Item* d_items = nullptr;
   SimDev* d_sims = nullptr;
@@ -504,6 +505,51 @@ extern "C" int sph_autotune(sph_sim* s, double* best_ms, double* best_cpp) {
       bcpp = cpp;
     }
   }
+  // then the walk's tile shape on that grid, one knob at a time (sub-tiles,
+  // first lane group, split threshold); the bracket width stays: it answers
+  // to the motion between passes, which passes at rest cannot show
+  if (!rc && !s->bitwise) {
+    if ((rc = fetch_sims(s)) == 0) {
+      layout_cells(s, bcpp);
+      if (hipMemcpyAsync(s->pb.sims, s->hsims.data(), s->hsims.size() * sizeof(SimDev),
+                         hipMemcpyHostToDevice, s->st) != hipSuccess)
+        rc = fail(SPH_EDEVICE, "upload failed");
+    }
+    auto timed_pass = [&](float* ms) -> int {
+      int r = run_pass(s, kPermInit, kForceNoKick);  // sets h on this grid
+      if (r) return r;
+      (void)hipEventRecord(e0, s->st);  // a failed record shows in the sync
+      if ((r = run_pass(s, kPermInit, kForceNoKick))) return r;
+      (void)hipEventRecord(e1, s->st);
+      if (hipEventSynchronize(e1) != hipSuccess) return fail(SPH_EDEVICE, "autotune pass failed");
+      if (hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+        return fail(SPH_EDEVICE, "autotune timing failed");
+      return 0;
+    };
+    WalkKnobs bk = s->pb.walk;
+    float bms = 0.f;
+    if (!rc) rc = timed_pass(&bms);
+    const int subs[] = {8, 4}, groups[] = {64, 32}, splits[] = {24, 32, 16};
+    for (int knob = 0; knob < 3 && !rc; ++knob) {
+      const int* vals = knob == 0 ? subs : (knob == 1 ? groups : splits);
+      const int nv = knob == 2 ? 3 : 2;
+      for (int i = 0; i < nv && !rc; ++i) {
+        WalkKnobs kk = bk;
+        int& slot = knob == 0 ? kk.nsub : (knob == 1 ? kk.group : kk.split);
+        if (slot == vals[i]) continue;
+        slot = vals[i];
+        s->pb.walk = kk;
+        float ms = 0.f;
+        if ((rc = timed_pass(&ms))) break;
+        if (ms < bms) {
+          bms = ms;
+          bk = kk;
+        }
+      }
+    }
+    s->pb.walk = bk;
+    if (!rc) best = bms;
+  }
   s->timing = timing;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -517,6 +563,44 @@ extern "C" int sph_autotune(sph_sim* s, double* best_ms, double* best_cpp) {
   if (best_ms) *best_ms = best;
   if (best_cpp) *best_cpp = bcpp;
   return check_device_flags(s);
+}
+
+static bool valid_knobs(const WalkKnobs& k) {
+  return k.delta >= 0.005 && k.delta <= 0.1 && k.split >= 8 && k.split <= 64 &&
+         (k.group == 16 || k.group == 32 || k.group == 64) && (k.nsub == 4 || k.nsub == 8);
+}
+
+extern "C" int sph_get_tuning(sph_sim* s, double out[5]) {
+  if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  out[0] = s->grid_cpp;
+  out[1] = s->pb.walk.delta;
+  out[2] = s->pb.walk.split;
+  out[3] = s->pb.walk.group;
+  out[4] = s->pb.walk.nsub;
+  return 0;
+}
+
+extern "C" int sph_set_tuning(sph_sim* s, const double in[5]) {
+  if (!s || !in) return fail(SPH_EINVAL, "null argument");
+  bool known_grid = false;
+  for (double cpp : kTuneCpp) known_grid = known_grid || cpp == in[0];
+  WalkKnobs k = kWalkDefault;
+  k.delta = in[1];
+  k.split = (int32_t)in[2];
+  k.group = (int32_t)in[3];
+  k.nsub = (int32_t)in[4];
+  if (!known_grid || !valid_knobs(k) || k.split != in[2] || k.group != in[3] || k.nsub != in[4])
+    return fail(SPH_EINVAL, "tuning out of range (grid 0.5..16 cells per particle, delta "
+                            "0.005..0.1, split 8..64, group 16/32/64, sub-tiles 4/8)");
+  int rc = fetch_sims(s);  // waits for the handle's work; keeps device scalars
+  if (rc) return rc;
+  s->drop_graphs();
+  layout_cells(s, in[0]);
+  s->pb.walk = k;
+  HIPCHK(hipMemcpyAsync(s->pb.sims, s->hsims.data(), s->hsims.size() * sizeof(SimDev),
+                        hipMemcpyHostToDevice, s->st));
+  HIPCHK(hipStreamSynchronize(s->st));
+  return 0;
 }
 
 extern "C" int sph_synchronize(sph_sim* s) {
@@ -766,12 +850,15 @@ struct CkptHeader {
   uint32_t header_bytes;
   int64_t n;
   int64_t step;
-  int32_t have_h;
+  uint8_t have_h;
+  uint8_t walk_split;  // the walk's tile shape (zero in images written
+  uint8_t walk_group;  // before it was kept: the defaults)
+  uint8_t walk_nsub;
   uint32_t params_bytes;
   uint32_t simdev_bytes;
   uint32_t nfields;
   double grid_cpp;
-  uint64_t reserved;
+  double walk_delta;
 };
 static_assert(sizeof(CkptHeader) == 64, "checkpoint header is 64 bytes");
 
@@ -847,11 +934,15 @@ extern "C" int sph_checkpoint_save(sph_sim* s, void* buf, int64_t bytes) {
   hd.header_bytes = sizeof hd;
   hd.n = n;
   hd.step = s->step;
-  hd.have_h = s->pb.have_h;
+  hd.have_h = (uint8_t)s->pb.have_h;
   hd.params_bytes = sizeof(sph_params);
   hd.simdev_bytes = sizeof(SimDev);
   hd.nfields = kCkptFields;
   hd.grid_cpp = s->grid_cpp;
+  hd.walk_delta = s->pb.walk.delta;
+  hd.walk_split = (uint8_t)s->pb.walk.split;
+  hd.walk_group = (uint8_t)s->pb.walk.group;
+  hd.walk_nsub = (uint8_t)s->pb.walk.nsub;
   char* p = (char*)buf;
   std::memcpy(p, &hd, sizeof hd);
   p += sizeof hd;
@@ -891,6 +982,14 @@ extern "C" int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out
   bool known_grid = false;
   for (double cpp : kTuneCpp) known_grid = known_grid || cpp == hd.grid_cpp;
   if (!known_grid) return fail(SPH_EINVAL, "corrupt checkpoint header (grid)");
+  WalkKnobs knobs = kWalkDefault;
+  if (hd.walk_nsub != 0) {
+    knobs.delta = hd.walk_delta;
+    knobs.split = hd.walk_split;
+    knobs.group = hd.walk_group;
+    knobs.nsub = hd.walk_nsub;
+    if (!valid_knobs(knobs)) return fail(SPH_EINVAL, "corrupt checkpoint header (walk shape)");
+  }
   const char* p = (const char*)buf + sizeof hd;
   sph_params prm;
   std::memcpy(&prm, p, sizeof prm);
@@ -929,6 +1028,7 @@ extern "C" int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out
     return code;
   };
   layout_cells(h, hd.grid_cpp);
+  h->pb.walk = knobs;
   take_device_scalars(h->hsims[0], sd);
   double* a[kCkptFields];
   ckpt_arrays(h, a);
@@ -994,7 +1094,8 @@ extern "C" int sph_append(sph_sim* s, int64_t m, const double* x, const double* 
     sph_destroy(h);
     return code;
   };
-  layout_cells(h, s->grid_cpp);  // same cells per particle as before
+  layout_cells(h, s->grid_cpp);  // same cells per particle and walk shape as before
+  h->pb.walk = s->pb.walk;
   // device-updated scalars carry over (the next half step was set by the
   // last pass, over the particles that had neighbours)
   take_device_scalars(h->hsims[0], s->hsims[0]);

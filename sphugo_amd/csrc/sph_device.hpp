@@ -161,6 +161,20 @@ struct Item {
   int64_t first;
 };
 
+// Tile-search shape of the kNN walk (the autotuner's knobs): bracket
+// half-width around each query's last h, the largest candidate count per
+// query of a lane group before it is halved, the lane group a wave starts
+// with (queries searched together: 64, 32 or 16), and the sub-tiles a wave
+// filters its staged candidates for (4 or 8).
+struct WalkKnobs {
+  double delta;
+  int32_t split;
+  int32_t group;
+  int32_t nsub;
+  int32_t pad;
+};
+constexpr WalkKnobs kWalkDefault = {0.025, 24, 64, 8, 0};
+
 struct KnnStats {
   unsigned long long retries;         // searches needing a second radius
   unsigned long long ties;            // exact d2 ties at the k-th boundary

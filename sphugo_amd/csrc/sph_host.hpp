@@ -68,6 +68,7 @@ struct PassBuffers {
   uint32_t* knn_ann;       // per query: annulus candidates (kAnn entries)
   int have_h;             // every particle has an h from an earlier pass
   int any_open;           // some simulation has an open axis (k_bbox, k_box run)
+  WalkKnobs walk;         // tile-search shape of the kNN walk
   int* err;             // device error flags: [0] any, [1 + s] simulation s (1: a
                         // particle left the images, 2: kNN found fewer than k)
   SortBuffers sort;

@@ -480,6 +480,7 @@ extern "C" int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kma
   while ((1LL << bits) < pb.ncells) ++bits;
   pb.key_bits = bits;
   pb.hole_cpp = kHoleCpp;
+  pb.walk = kWalkDefault;
   pb.have_h = c.bitwise ? 0 : 1;  // jobs without h take the exact search
   c.cell_capacity = pb.ncells + 1;
   SimDev* d_sims = nullptr;

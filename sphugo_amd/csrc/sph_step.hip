@@ -324,10 +324,6 @@ __device__ inline void cell_range(const SimDev* S, const uint32_t* __restrict__ 
 
 constexpr int kV1Cells = 128;  // coarse cells of one search (<= 9 x 9 used)
 constexpr int kTileCells = 256;  // coarse cells of one tile box (<= 15 x 15 used)
-constexpr double kDelta = 0.025;  // bracket half-width around the last h
-constexpr int kSplit = 24;       // max tile candidates per query before a split
-constexpr int kSub = 8;          // sub-tiles of a wave (8 queries each)
-constexpr int kSubW = 64 / kSub;
 
 // Per-wave scratch of the exact wavefront search.
 struct SearchShared {
@@ -342,7 +338,8 @@ struct SearchShared {
   double fterm[kMaxK];
 };
 
-// Per-wave scratch of the tile search (lane = query).
+// Per-wave scratch of the tile search (lane = query), KSUB sub-tiles.
+template <int KSUB>
 struct TileShared {
   uint32_t pst[kTileCells];
   int ppref[kTileCells + 1];
@@ -350,16 +347,11 @@ struct TileShared {
   int8_t pty[kTileCells];
   double2 sxy[65];           // staged candidates; entry 64 is a NaN sentinel
   uint32_t spk[65];
-  double sbb[kSub][4];       // box of each sub-tile (x0, x1, y0, y1)
-  uint8_t slist[kSub][64];   // staged candidates within reach of each sub-tile
-  uint32_t slpk[kSub][64];   // their packed entries, in the same order
-  unsigned long long smask[kSub];  // staged slots in each sub-tile's list
+  double sbb[KSUB][4];       // box of each sub-tile (x0, x1, y0, y1)
+  uint8_t slist[KSUB][64];   // staged candidates within reach of each sub-tile
+  uint32_t slpk[KSUB][64];   // their packed entries, in the same order
+  unsigned long long smask[KSUB];  // staged slots in each sub-tile's list
   unsigned long long selfm[64];    // per query: staged slots that are itself
-};
-
-union WaveShared {
-  SearchShared s;
-  TileShared t;
 };
 
 struct SearchOut {
@@ -793,7 +785,7 @@ __device__ inline double local_radius(const SimDev* S, const uint32_t* __restric
 // Five waves per SIMD (a few spilled registers): the walk is issue-bound
 // with long LDS and gather latencies; its LDS (about 7 KB per wave) allows
 // five blocks per CU.
-template <bool BITWISE, bool FP32C>
+template <bool BITWISE, bool FP32C, int KSUB>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k_knn(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const double2* __restrict__ XY,
@@ -803,14 +795,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     uint32_t* __restrict__ nbr, int kstride, double* __restrict__ RHO,
     double* __restrict__ C, void* __restrict__ recp, KnnStats* __restrict__ stats,
     uint2* __restrict__ retry, uint32_t* __restrict__ retry_count,
-    uint32_t* __restrict__ cnt, uint32_t* __restrict__ ann) {
-  __shared__ TileShared wsh[4];
+    uint32_t* __restrict__ cnt, uint32_t* __restrict__ ann, WalkKnobs kn) {
+  constexpr int kSub = KSUB;       // sub-tiles of a wave
+  constexpr int kSubW = 64 / KSUB; // consecutive queries per sub-tile
+  __shared__ TileShared<KSUB> wsh[4];
   const int item_idx = xcd_item(nitems);
   const Item it = items[item_idx];
   const SimDev* S = sims + it.sim;
   const int wv = threadIdx.x >> 6;
   const int lane = threadIdx.x & 63;
-  TileShared& ts = wsh[wv];
+  TileShared<KSUB>& ts = wsh[wv];
   const int k = S->k;
   const double lx = S->lx, ly = S->ly;
   unsigned long long nretry = 0, nties = 0, nfail = 0, nfall = 0;
@@ -827,7 +821,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   bool resolved = false;
   double tau = 0.0;
   if constexpr (!BITWISE) {
-    const double Rlo = hp * (1 - kDelta), Rhi = hp * (1 + kDelta);
+    const double Rlo = hp * (1 - kn.delta), Rhi = hp * (1 + kn.delta);
     const double Rlo2 = Rlo * Rlo, Rhi2 = Rhi * Rhi;
     const double Rm = Rhi * (1 + 1e-12) + 1e-15 * (lx + ly);
     int c_lo = 0, c_an = 0;
@@ -840,11 +834,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     // index fill the same lines
     uint32_t* annrow = ann + (t >> 6) * (int64_t)(64 * kAnn) + (t & 63);
     // Lane groups: aligned blocks of consecutive queries.  A block whose
-    // box holds more than kSplit candidates per query is halved (cell
+    // box holds more than kn.split candidates per query is halved (cell
     // order can put two distant regions in one wave); blocks below 8 lanes
     // go to the wavefront search, which handles them at lower cost.
     unsigned long long pending = ballot(fast);
-    int gsz = 64;
+    const int gsz0 = kn.group;
+    int gsz = gsz0;
     while (pending) {
       const int l0 = __builtin_ctzll(pending);
       const int ga = l0 & ~(gsz - 1);
@@ -896,7 +891,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
       npieces = __builtin_amdgcn_readfirstlane(npieces);
       ncand = __builtin_amdgcn_readfirstlane(ncand);
       const int gs = __popcll(gm);
-      if (ncand > kSplit * gs) {
+      if (ncand > kn.split * gs) {
         if (gsz > 8) {
           gsz >>= 1;
         } else {
@@ -905,7 +900,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
             ++nsplitfb;
           }
           pending &= ~gm;
-          gsz = 64;
+          gsz = gsz0;
         }
         __builtin_amdgcn_wave_barrier();
         continue;
@@ -1109,7 +1104,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         __builtin_amdgcn_wave_barrier();
       }
       pending &= ~gm;
-      gsz = 64;
+      gsz = gsz0;
     }
     // hand the resolvable queries to k_knn_finish: certain neighbours are
     // in the row, annulus candidates go to `ann`
@@ -1818,11 +1813,11 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
     // that drew quick searches make room for others (a few % faster than
     // one resident round)
     const int fb_grid = fastpath ? 4096 : (b.nitems < 4096 ? b.nitems : 4096);
-#define SPH_KNN(F32)                                                                          \
-  hipLaunchKernelGGL((k_knn<false, F32>), grd, dim3(256), 0, st, b.items, b.nitems, b.sims,    \
-                     b.xyp, b.wrap, b.cstart, b.fnew.h, b.fnew.ep,                              \
+#define SPH_KNN(F32, NSUB)                                                                    \
+  hipLaunchKernelGGL((k_knn<false, F32, NSUB>), grd, dim3(256), 0, st, b.items, b.nitems,      \
+                     b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h, b.fnew.ep,                      \
                      b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr, b.kmax, b.fnew.rho, b.fnew.c,    \
-                     b.rec, b.stats, b.retry, b.retry_count, b.knn_cnt, b.knn_ann)
+                     b.rec, b.stats, b.retry, b.retry_count, b.knn_cnt, b.knn_ann, b.walk)
 #define SPH_FB(BW, F32)                                                                       \
   hipLaunchKernelGGL((k_knn_fallback<BW, F32>), dim3(fb_grid), dim3(256), 0, st, b.items,     \
                      b.nitems, b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h,                       \
@@ -1830,7 +1825,12 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
                      b.fnew.c, b.rec, b.stats, b.retry, b.retry_count, fastpath ? 0 : 1, b.err)
     mark(6, 0);
     if (fastpath) {
-      if (b.prec == kPrecFP32C) SPH_KNN(true); else SPH_KNN(false);
+      const bool f32 = b.prec == kPrecFP32C;
+      if (b.walk.nsub == 4) {
+        if (f32) SPH_KNN(true, 4); else SPH_KNN(false, 4);
+      } else {
+        if (f32) SPH_KNN(true, 8); else SPH_KNN(false, 8);
+      }
     }
     mark(6, 1);
     mark(7, 0);

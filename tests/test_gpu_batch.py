@@ -126,3 +126,37 @@ def test_autotune_plain_forms_within_contract(precision):
         for f in ("ax", "ay", "edot"):
             fa, fb = a.get(f), b.get(f)
             assert np.abs(fa - fb).max() <= 1e-12 * np.abs(fb).max(), f
+
+
+@pytest.mark.parametrize("tuning", [dict(subtiles=4), dict(group=32), dict(split=16),
+                                    dict(delta=0.02), dict(cells_per_particle=1.0, subtiles=4,
+                                                           group=32, split=32, delta=0.05)])
+def test_walk_shapes_match_oracle(tuning):
+    """Every tile shape the autotuner (or a caller) may set finds the same
+    exact neighbour sets: one pass after steps (the walk) against the oracle."""
+    import oracle as O
+    from test_gpu_parity import _check_forces, _check_knn, state_pass_vs_oracle
+    st = ics.clustered(20000, seed=23, k=64)
+    p = _p(st, k=64)
+    g = L.Sim(p, st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    L.set_tuning(g, **tuning)
+    got = L.get_tuning(g)
+    for key, v in tuning.items():
+        assert got[key] == v, key
+    g.step(3)
+    op = O.default_params(knn_mode=O.KNN_EXACT, use_cfl=1, k=64, mass=st["mass"], gamma=5 / 3,
+                          cfl=0.3, hor=(0, 1), vert=(0, 1), visc_alpha=1.0, visc_beta=2.0)
+    o, nfb = state_pass_vs_oracle(g, op)
+    # the walk resolves most queries (a wide bracket overflows more annuli)
+    assert nfb < 0.5 * st["n"], nfb
+    _check_knn(g, o)
+    _check_forces(g, o)
+
+
+def test_set_tuning_rejects():
+    st = ics.uniform(3000, seed=2)
+    a = L.Sim(_p(st), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    for bad in (dict(subtiles=6), dict(group=48), dict(split=4), dict(delta=0.5),
+                dict(cells_per_particle=3.0)):
+        with pytest.raises(L.SphError):
+            L.set_tuning(a, **bad)

@@ -83,3 +83,42 @@ def test_checkpoint_rejects_batches_and_bad_images():
     bad[0:8] = b"XXXXXXXX"
     with pytest.raises(L.SphError, match="magic"):
         L.Sim.from_checkpoint(bytes(bad))
+
+
+def test_resume_keeps_the_walk_shape():
+    st = ics.clustered(6000, seed=14, k=64)
+    a = _sim(st, k=64)
+    L.set_tuning(a, subtiles=4, group=32, split=16, delta=0.02)
+    a.step(2)
+    b = L.Sim.from_checkpoint(a.checkpoint())
+    assert L.get_tuning(b) == L.get_tuning(a)
+    a.step(2)
+    b.step(2)
+    _same(a, b)
+
+
+def test_load_refuses_an_edited_simulation_record():
+    """The simulation record in an image must be the one its parameters
+    give: an edited grid size (gx) is refused with SPH_EINVAL before any
+    device work, as is an edited box."""
+    import ctypes
+    import struct
+    st = ics.uniform(3000, seed=4)
+    a = _sim(st)
+    a.step(1)
+    img = bytearray(a.checkpoint())
+    base = 64 + ctypes.sizeof(L.Params)  # header, parameters, then the record
+    gx_at = base + 8 + 8 + 6 * 4         # off, coff, n k kernel prec flags use_cfl
+    bad = bytearray(img)
+    struct.pack_into("<i", bad, gx_at, struct.unpack_from("<i", img, gx_at)[0] * 2)
+    with pytest.raises(L.SphError, match="simulation record"):
+        L.Sim.from_checkpoint(bytes(bad))
+    lx_at = gx_at + 5 * 4 + 4 + 2 * 8    # gx gy lg pmode fmode open, x0 y0 -> lx
+    bad = bytearray(img)
+    struct.pack_into("<d", bad, lx_at, 2.0)
+    with pytest.raises(L.SphError, match="simulation record"):
+        L.Sim.from_checkpoint(bytes(bad))
+    b = L.Sim.from_checkpoint(bytes(img))  # the unedited image still loads
+    b.step(1)
+    a.step(1)
+    _same(a, b)
