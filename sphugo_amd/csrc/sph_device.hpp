@@ -248,24 +248,25 @@ __device__ inline double kern_DF_fma(int32_t kernel, double q) {
   const double a = 1 - q;
   return -(a * a);
 }
-// FP32 forms (FP32C), fused multiply-adds where they fit
+// FP32 forms (FP32C), fused multiply-adds where they fit.  The two pieces
+// of the cubic spline are both evaluated and one is selected: the lanes of
+// a wave take both sides anyway, and a select needs no exec-mask branch.
 __device__ inline float kern_Ff(int32_t kernel, float q) {
+  const float a = 1.f - q;
   if (kernel == kWendland) {
-    float a = 1.f - q, a2 = a * a;
+    const float a2 = a * a;
     return a2 * a2 * __builtin_fmaf(4.f, q, 1.f);
   }
-  if (q < 0.5f) return __builtin_fmaf(q * q, q - 1.f, 1.f / 6.f);
-  float a = 1.f - q;
-  return a * a * a * (1.f / 3.f);
+  const float lo = __builtin_fmaf(q * q, q - 1.f, 1.f / 6.f);
+  const float hi = a * a * a * (1.f / 3.f);
+  return q < 0.5f ? lo : hi;
 }
 __device__ inline float kern_DFf(int32_t kernel, float q) {
-  if (kernel == kWendland) {
-    float a = 1.f - q;
-    return -10.f * q * a * a * a;
-  }
-  if (q < 0.5f) return q * __builtin_fmaf(3.f, q, -2.f);
-  float a = 1.f - q;
-  return -a * a;
+  const float a = 1.f - q;
+  if (kernel == kWendland) return -10.f * q * a * a * a;
+  const float lo = q * __builtin_fmaf(3.f, q, -2.f);
+  const float hi = -a * a;
+  return q < 0.5f ? lo : hi;
 }
 
 // FP64 reciprocal and reciprocal square root from the hardware estimates

@@ -1200,9 +1200,14 @@ __device__ inline void stage_window(T* lds, const T* __restrict__ g, int64_t wb,
   }
 }
 
-template <int N, bool NT = false, typename S, typename Body>
+struct NoFlush {
+  __device__ void operator()() const {}
+};
+
+// `flush` runs after each round of N entries (block-compensated sums).
+template <int N, bool NT = false, typename S, typename Body, typename Flush = NoFlush>
 __device__ inline void row_visit(const NbrRow row, int k, uint32_t self, const S& src,
-                                 Body body) {
+                                 Body body, Flush flush = Flush()) {
   static_assert(N == 4 || N == 8, "one or two row chunks per round");
   using T = std::remove_cv_t<std::remove_reference_t<decltype(src[0u])>>;
   auto ld = [&](int c) {
@@ -1228,8 +1233,27 @@ __device__ inline void row_visit(const NbrRow row, int k, uint32_t self, const S
       if (n0 + u >= k) break;
       body(pks[u], pj[u]);
     }
+    flush();
   }
 }
+
+// Block-compensated FP32 sum: the terms of one round (4 or 8 neighbours)
+// add up plainly, and each round's partial joins the running sum through a
+// Kahan step.  Besides the compensated sum's own error this adds at most
+// (B - 1) 2^-24 of the sum of |terms| of a block of B, against four
+// operations per term for a Kahan step per term.
+struct BlockKahan {
+  float s = 0.f, c = 0.f, p = 0.f;
+  __device__ void add(float t) { p += t; }
+  __device__ void flush() {
+    const float y = p - c;
+    const float tt = s + y;
+    c = (tt - s) - y;
+    s = tt;
+    p = 0.f;
+  }
+  __device__ double value() const { return (double)s - (double)c; }
+};
 
 // Completes the fast-path queries: the r = k - c_lo smallest annulus
 // candidates by (d2, ID, image) fill the row, then density, sound speed and
@@ -1360,14 +1384,9 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     const int kern = S->kernel;
     double facc;
     if constexpr (FP32C) {
-      float s_f = 0.f, c_f = 0.f;
-      auto kahan = [&](float q) {
-        const float f = kern_Ff(kern, __builtin_fminf(q, 1.f));
-        const float y = f - c_f;
-        const float tt = s_f + y;
-        c_f = (tt - s_f) - y;
-        s_f = tt;
-      };
+      BlockKahan acc;
+      auto kahan = [&](float q) { acc.add(kern_Ff(kern, __builtin_fminf(q, 1.f))); };
+      auto flush = [&]() { acc.flush(); };
       // displacements from the fixed-point positions (the minimum image,
       // exact to L / 2^32) while every neighbour lies within half a box,
       // scaled straight to units of h; otherwise from the FP64 positions.
@@ -1382,7 +1401,7 @@ __global__ __launch_bounds__(256) void k_knn_finish(
           const float ex = __builtin_fmaf((float)(int32_t)(pj.x - me.x), gx, cx);
           const float ey = __builtin_fmaf((float)(int32_t)(pj.y - me.y), gy, cy);
           kahan(__builtin_amdgcn_sqrtf(__builtin_fmaf(ey, ey, ex * ex)));
-        });
+        }, flush);
       } else {
         // q^2 in FP64 before the FP32 square root (see wave_search)
         const double itau = 1.0 / tau;
@@ -1390,9 +1409,9 @@ __global__ __launch_bounds__(256) void k_knn_finish(
           const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
           const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
           kahan(__builtin_amdgcn_sqrtf((float)((ex * ex + ey * ey) * itau)));
-        });
+        }, flush);
       }
-      facc = (double)s_f - (double)c_f;
+      facc = acc.value();
     } else {
       double acc = 0.0;
       const double hinv64 = 1.0 / h;
@@ -1580,8 +1599,13 @@ __global__ __launch_bounds__(256) void k_force(
     const float vpxi = me.vpx, vpyi = me.vpy, cif = me.c, rhoif = me.rho, hif = me.h, Af = me.B;
     const float hinv = __builtin_amdgcn_rcpf(2.0f * hif);
     const float alpha = (float)S->alpha, beta = (float)S->beta, eta2 = (float)S->eta2;
-    // Kahan-compensated sums
-    float fx = 0.f, fy = 0.f, fe = 0.f, cx = 0.f, cy = 0.f, ce = 0.f;
+    // block-compensated sums (a Kahan step per round of 4 neighbours)
+    BlockKahan ax_, ay_, ae_;
+    auto flush = [&]() {
+      ax_.flush();
+      ay_.flush();
+      ae_.flush();
+    };
     auto pair = [&](float rx, float ry, const RecF& nb) {
 #pragma clang fp contract(fast)
       const float d2 = rx * rx + ry * ry;
@@ -1596,29 +1620,22 @@ __global__ __launch_bounds__(256) void k_force(
       const float dR = kern_DFf(kern, q);
       const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
       const float dot = vx_ * rx + vy_ * ry;
-      float piAB = 0.f;
-      if (dot < 0.f) {
-        const float cAB = cif + nb.c;
-        const float rhoAB = rhoif + nb.rho;
-        const float hAB = hif + nb.h;
-        // mu = dot hAB / (r^2 + eta2), pi = mu (beta mu - alpha cAB) / rhoAB,
-        // with one reciprocal of (r^2 + eta2) rhoAB
-        const float a_ = dot * hAB;
-        const float inv = __builtin_amdgcn_rcpf((d2 + eta2) * rhoAB);
-        const float muAB = a_ * inv * rhoAB;
-        piAB = __builtin_fmaf(beta, muAB, -alpha * cAB) * (a_ * inv);
-      }
+      // viscosity where the pair approaches (dot < 0): evaluated for every
+      // pair and selected, since the lanes of a wave take both sides anyway
+      // mu = dot hAB / (r^2 + eta2), pi = mu (beta mu - alpha cAB) / rhoAB,
+      // with one reciprocal of (r^2 + eta2) rhoAB
+      const float cAB = cif + nb.c;
+      const float rhoAB = rhoif + nb.rho;
+      const float hAB = hif + nb.h;
+      const float a_ = dot * hAB;
+      const float inv = __builtin_amdgcn_rcpf((d2 + eta2) * rhoAB);
+      const float muAB = a_ * inv * rhoAB;
+      const float pv = __builtin_fmaf(beta, muAB, -alpha * cAB) * (a_ * inv);
+      const float piAB = dot < 0.f ? pv : 0.f;
       const float w = (piAB + Af + nb.B) * dR * dinv;
-      const float tx = rx * w, ty = ry * w, te = dot * dR;
-      float y_ = tx - cx, tt = fx + y_;
-      cx = (tt - fx) - y_;
-      fx = tt;
-      y_ = ty - cy; tt = fy + y_;
-      cy = (tt - fy) - y_;
-      fy = tt;
-      y_ = te - ce; tt = fe + y_;
-      ce = (tt - fe) - y_;
-      fe = tt;
+      ax_.add(rx * w);
+      ay_.add(ry * w);
+      ae_.add(dot * dR);
     };
     // 4 records (32 B each) in flight per lane: with most of them in the LDS
     // window, the registers of more buy less than a fifth wave per SIMD.  A
@@ -1628,7 +1645,7 @@ __global__ __launch_bounds__(256) void k_force(
       row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t, const RecF& nb) {
         pair(__builtin_fmaf((float)(int32_t)(nb.ux - me.ux), fsx, csx),
              __builtin_fmaf((float)(int32_t)(nb.uy - me.uy), fsy, csy), nb);
-      });
+      }, flush);
     } else {
       row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t pk, const RecF& nb) {
         float rx, ry;
@@ -1642,11 +1659,11 @@ __global__ __launch_bounds__(256) void k_force(
           ry = (float)((pj.y - oy) - yi);
         }
         pair(rx, ry, nb);
-      });
+      }, flush);
     }
-    sx = (double)fx - (double)cx;
-    sy = (double)fy - (double)cy;
-    se = (double)fe - (double)ce;
+    sx = ax_.value();
+    sy = ay_.value();
+    se = ae_.value();
   }
   double r_e = 0, r_rho = 0, r_E = 0, r_px = 0, r_py = 0, r_cfl = INFINITY;
   if (valid) {
