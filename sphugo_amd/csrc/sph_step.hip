@@ -807,8 +807,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   TileShared<KSUB>& ts = wsh[wv];
   const int k = S->k;
   const double lx = S->lx, ly = S->ly;
-  unsigned long long nretry = 0, nties = 0, nfail = 0, nfall = 0;
-  unsigned long long nmiss_lo = 0, nmiss_hi = 0, nmiss_ann = 0, nsplitfb = 0;
+  // per-lane outcomes, counted with ballots at the end (each lane once)
+  bool miss_lo = false, miss_hi = false, miss_ann = false, split_fb = false;
 
   const int qi = wv * 64 + lane;
   const bool valid = qi < it.count;
@@ -897,7 +897,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
         } else {
           if (grp) {
             fast = false;
-            ++nsplitfb;
+            split_fb = true;
           }
           pending &= ~gm;
           gsz = gsz0;
@@ -1109,9 +1109,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     // hand the resolvable queries to k_knn_finish: certain neighbours are
     // in the row, annulus candidates go to `ann`
     if (fast) {
-      if (c_lo >= k) ++nmiss_lo;
-      else if (c_an > kAnn) ++nmiss_ann;
-      else if (c_lo + c_an < k) ++nmiss_hi;
+      miss_lo = c_lo >= k;
+      miss_ann = !miss_lo && c_an > kAnn;
+      miss_hi = !miss_lo && !miss_ann && c_lo + c_an < k;
     }
     if (fast && c_lo < k && c_an <= kAnn && c_lo + c_an >= k) {
       // the last c_lo % 4 certain entries sit at the top of the buffer
@@ -1134,37 +1134,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     if (valid) cnt[t] = resolved ? ((uint32_t)c_lo | ((uint32_t)c_an << 8)) : 0xFFFFFFFFu;
   }
   // misses go to the retry list, searched by k_knn_fallback
-  {
-    const bool need = valid && !resolved;
-    const unsigned long long m = ballot(need);
-    if (m) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(retry_count, (uint32_t)__popcll(m));
-      base = readlane_u(base, 0);
-      if (need) retry[base + lane_prefix(m)] = make_uint2((uint32_t)t, (uint32_t)item_idx);
-      const unsigned long long mh = m & ballot(hp > 0);
-      if (lane == 0) nfall += __popcll(mh);
-    }
+  const bool need = valid && !resolved;
+  const unsigned long long m = ballot(need);
+  if (m) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(retry_count, (uint32_t)__popcll(m));
+    base = readlane_u(base, 0);
+    if (need) retry[base + lane_prefix(m)] = make_uint2((uint32_t)t, (uint32_t)item_idx);
   }
-  nretry += nfall;
-  for (int o = 32; o > 0; o >>= 1) {
-    nretry += __shfl_xor(nretry, o, 64);
-    nties += __shfl_xor(nties, o, 64);
-    nfail += __shfl_xor(nfail, o, 64);
-    nmiss_lo += __shfl_xor(nmiss_lo, o, 64);
-    nmiss_hi += __shfl_xor(nmiss_hi, o, 64);
-    nmiss_ann += __shfl_xor(nmiss_ann, o, 64);
-    nsplitfb += __shfl_xor(nsplitfb, o, 64);
-  }
+  // counters: queries with an h that needed the exact search, and why
+  const unsigned long long nfall = __popcll(m & ballot(hp > 0));
+  const unsigned long long nlo = __popcll(ballot(miss_lo)), nhi = __popcll(ballot(miss_hi));
+  const unsigned long long nann = __popcll(ballot(miss_ann)), nsplit = __popcll(ballot(split_fb));
   if (lane == 0) {
     KnnStats* sp = stat_slot(stats);
-    stat_add(&sp->retries, nretry);
-    stat_add(&sp->ties, nties);
-    stat_add(&sp->failures, nfail);
-    stat_add(&sp->miss_lo, nmiss_lo);
-    stat_add(&sp->miss_hi, nmiss_hi);
-    stat_add(&sp->miss_ann, nmiss_ann);
-    stat_add(&sp->split_fallback, nsplitfb);
+    stat_add(&sp->retries, nfall);
+    stat_add(&sp->miss_lo, nlo);
+    stat_add(&sp->miss_hi, nhi);
+    stat_add(&sp->miss_ann, nann);
+    stat_add(&sp->split_fallback, nsplit);
   }
 }
 
