@@ -341,6 +341,34 @@ def error_report():
     return out
 
 
+def profile_step(precision: str, steps: int) -> None:
+    """A fixed workload to run under rocprofv3 counters: the clustered 4M
+    case (k = 64) created, one untimed step, then `steps` steps, every pass
+    launching each kernel once."""
+    st = ics.clustered(4194304, seed=3, k=64)
+    sim = L.Sim(params_for(st, L.PREC_FP32C if precision == "fp32c" else L.PREC_FP64),
+                st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    sim.step(1)
+    sim.step(steps)
+    print(f"stepped {steps} steps at N={st['n']} ({precision}); E = {sim.total_energy():.12g}",
+          flush=True)
+    sim.close()
+
+
+def profile_copy(mib: int) -> None:
+    """One device-to-device copy of known size: the reference that scales
+    the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(f"copied {mib} MiB (read {mib} MiB, wrote {mib} MiB)", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -364,7 +392,17 @@ def main():
     ap.add_argument("--serve-capacity", type=int, default=1 << 23, help="pool particle slots")
     ap.add_argument("--serve-only", action="store_true",
                     help="run only the serving workload and print its object")
+    ap.add_argument("--profile-step", choices=["fp32c", "fp64"],
+                    help="run the clustered 4M case for counter runs (no timing)")
+    ap.add_argument("--profile-copy", type=int, metavar="MIB",
+                    help="one device copy of MIB MiB (counter scaling)")
     args = ap.parse_args()
+    if args.profile_step:
+        profile_step(args.profile_step, args.steps)
+        return
+    if args.profile_copy:
+        profile_copy(args.profile_copy)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                           "(multi-GPU sharding is a design, not a run)"}))
