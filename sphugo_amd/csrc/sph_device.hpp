@@ -251,22 +251,45 @@ __device__ inline double kern_DF_fma(int32_t kernel, double q) {
 // FP32 forms (FP32C), fused multiply-adds where they fit.  The two pieces
 // of the cubic spline are both evaluated and one is selected: the lanes of
 // a wave take both sides anyway, and a select needs no exec-mask branch.
-__device__ inline float kern_Ff(int32_t kernel, float q) {
-  const float a = 1.f - q;
-  if (kernel == kWendland) {
+// FP32 forms of each kernel type.  The type is per simulation, so loops
+// over neighbours are instantiated per type (kern_visit) rather than
+// branching on it per pair.
+struct CubicF {
+  __device__ static float F(float q) {
+    const float a = 1.f - q;
+    const float lo = __builtin_fmaf(q * q, q - 1.f, 1.f / 6.f);
+    const float hi = a * a * a * (1.f / 3.f);
+    return q < 0.5f ? lo : hi;
+  }
+  __device__ static float DF(float q) {
+    const float a = 1.f - q;
+    const float lo = q * __builtin_fmaf(3.f, q, -2.f);
+    const float hi = -a * a;
+    return q < 0.5f ? lo : hi;
+  }
+};
+struct WendlandF {
+  __device__ static float F(float q) {
+    const float a = 1.f - q;
     const float a2 = a * a;
     return a2 * a2 * __builtin_fmaf(4.f, q, 1.f);
   }
-  const float lo = __builtin_fmaf(q * q, q - 1.f, 1.f / 6.f);
-  const float hi = a * a * a * (1.f / 3.f);
-  return q < 0.5f ? lo : hi;
+  __device__ static float DF(float q) {
+    const float a = 1.f - q;
+    return -10.f * q * a * a * a;
+  }
+};
+__device__ inline float kern_Ff(int32_t kernel, float q) {
+  return kernel == kWendland ? WendlandF::F(q) : CubicF::F(q);
 }
 __device__ inline float kern_DFf(int32_t kernel, float q) {
-  const float a = 1.f - q;
-  if (kernel == kWendland) return -10.f * q * a * a * a;
-  const float lo = q * __builtin_fmaf(3.f, q, -2.f);
-  const float hi = -a * a;
-  return q < 0.5f ? lo : hi;
+  return kernel == kWendland ? WendlandF::DF(q) : CubicF::DF(q);
+}
+// Calls f(CubicF{}) or f(WendlandF{}) for the simulation's kernel type.
+template <typename Fn>
+__device__ inline void kern_visit(int32_t kernel, Fn&& f) {
+  if (kernel == kWendland) f(WendlandF{});
+  else f(CubicF{});
 }
 
 // FP64 reciprocal and reciprocal square root from the hardware estimates

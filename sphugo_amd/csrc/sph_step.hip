@@ -1156,26 +1156,74 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   }
 }
 
-// Visits the k entries of a neighbour row, N (4 or 8) at a time: the N
-// sources (src[index]) are gathered together and the next N entries' row
-// chunks are loaded before the current N are processed, so that the row and
-// gather latencies overlap.  Entries past k gather the query's own slot
-// `self`.  NT: the row is read for the last time (non-temporal loads).
-// A source array read through a block's LDS window: entries [wb, wb + W)
+// Byte offset of record j of an array of T, from a packed entry
+// pk = j << 4 | image (pk & ~15 is 16 j): one shift at most, so that the
+// address is one shift-add on the array base.
+template <typename T>
+__device__ inline uint64_t rec_off(uint32_t pk) {
+  constexpr int sz = (int)sizeof(T);
+  static_assert(sz == 8 || sz == 16 || sz == 32 || sz == 64, "record size");
+  const uint32_t m = pk & ~15u;
+  if constexpr (sz == 8) return m >> 1;
+  else if constexpr (sz == 16) return m;
+  else return (uint64_t)m << (sz == 32 ? 1 : 2);
+}
+
+// Sources of the neighbour gathers, indexed by packed entry.  A gather runs
+// in three steps over a round's entries: near() issues the first load and
+// says whether it is the value, pin() waits for those loads, far() loads the
+// rest.  An array: one global load.
+template <typename T>
+struct ArrSrc {
+  const T* __restrict__ g;
+  __device__ bool near(uint32_t pk, T& v) const {
+    v = *(const T*)((const char*)g + rec_off<T>(pk));
+    return true;
+  }
+  __device__ void pin(T&) const {}
+  __device__ T far(uint32_t pk) const { return *(const T*)((const char*)g + rec_off<T>(pk)); }
+};
+
+// An array read through a block's LDS window: entries [wb, wb + W)
 // (indices mod 2^32) come from the staged copy, the rest from the array.
 // Particles are in Hilbert order, so most neighbours of a block's 256
-// consecutive queries lie within a few hundred slots of them.
+// consecutive queries lie within a few hundred slots of them.  Every lane
+// reads the window (slot 0 when outside it) and pin() passes the values
+// through an empty asm statement, so that the compiler cannot fold the two
+// loads into one flat load: a flat load completes out of order with the
+// global loads, and every wait for a gather would also wait for the row
+// chunk loaded behind it.
 template <typename T, int W>
 struct WinSrc {
   const T* __restrict__ g;
   const T* lds;
   uint32_t wb;
-  __device__ T operator[](uint32_t j) const {
-    const uint32_t wj = j - wb;
-    if (wj < (uint32_t)W) return lds[wj];
-    return g[j];
+  __device__ bool near(uint32_t pk, T& v) const {
+    const uint32_t wj = (pk >> 4) - wb;
+    const bool in = wj < (uint32_t)W;
+    v = lds[in ? wj : 0u];
+    return in;
   }
+  __device__ void pin(T& v) const {
+    uint32_t* w = (uint32_t*)&v;
+#pragma unroll
+    for (int i = 0; i < (int)(sizeof(T) / 4); ++i) asm("" : "+v"(w[i]));
+  }
+  __device__ T far(uint32_t pk) const { return *(const T*)((const char*)g + rec_off<T>(pk)); }
 };
+
+// Gathers the sources of n packed entries.
+template <int N, typename S, typename T>
+__device__ inline void gather(const S& src, const uint32_t* pks, T* pj) {
+  bool in[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) in[u] = src.near(pks[u], pj[u]);
+#pragma unroll
+  for (int u = 0; u < N; ++u) src.pin(pj[u]);
+#pragma unroll
+  for (int u = 0; u < N; ++u)
+    if (!in[u]) pj[u] = src.far(pks[u]);
+}
 
 // Stages entries [wb, wb + W) of `g` that lie in [lo, hi) into `lds` (the
 // block's threads together; the caller synchronises).
@@ -1192,12 +1240,18 @@ struct NoFlush {
   __device__ void operator()() const {}
 };
 
-// `flush` runs after each round of N entries (block-compensated sums).
+// Visits the k entries of a neighbour row, N (4 or 8) at a time: the N
+// sources (src's records of the entries) are gathered together and the next N entries' row
+// chunks are loaded before the current N are processed, so that the row and
+// gather latencies overlap.  Full rounds gather their entries as they are; a
+// last partial round gathers the query's own slot `self` for the entries past
+// k.  NT: the row is read for the last time (non-temporal loads).  `flush`
+// runs after each round of N entries (block-compensated sums).
 template <int N, bool NT = false, typename S, typename Body, typename Flush = NoFlush>
 __device__ inline void row_visit(const NbrRow row, int k, uint32_t self, const S& src,
                                  Body body, Flush flush = Flush()) {
   static_assert(N == 4 || N == 8, "one or two row chunks per round");
-  using T = std::remove_cv_t<std::remove_reference_t<decltype(src[0u])>>;
+  using T = std::remove_cv_t<std::remove_reference_t<decltype(src.far(0u))>>;
   auto ld = [&](int c) {
     if constexpr (NT) return __builtin_nontemporal_load(row.chunk(c));
     else return *row.chunk(c);
@@ -1207,15 +1261,27 @@ __device__ inline void row_visit(const NbrRow row, int k, uint32_t self, const S
   const u32x4 zero = {0u, 0u, 0u, 0u};
   u32x4 qa = ld(0);
   u32x4 qb = (N == 8 && k > 4) ? ld(1) : zero;
-  for (int n0 = 0; n0 < k; n0 += N) {
+  int n0 = 0;
+  for (; n0 + N <= k; n0 += N) {
     const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
     T pj[N];
-#pragma unroll
-    for (int u = 0; u < N; ++u) pj[u] = src[n0 + u < k ? (pks[u] >> 4) : self];
+    gather<N>(src, pks, pj);
     if (n0 + N < k) {
       qa = ld((n0 + N) >> 2);
       if constexpr (N == 8) qb = n0 + 12 < k ? ld(((n0 + N) >> 2) + 1) : zero;
     }
+#pragma unroll
+    for (int u = 0; u < N; ++u) body(pks[u], pj[u]);
+    flush();
+  }
+  if (n0 < k) {
+    const uint32_t spk = self << 4;
+    const uint32_t pks[8] = {qa.x, qa.y, qa.z, qa.w, qb.x, qb.y, qb.z, qb.w};
+    uint32_t gpk[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) gpk[u] = n0 + u < k ? pks[u] : spk;
+    T pj[N];
+    gather<N>(src, gpk, pj);
 #pragma unroll
     for (int u = 0; u < N; ++u) {
       if (n0 + u >= k) break;
@@ -1233,6 +1299,7 @@ __device__ inline void row_visit(const NbrRow row, int k, uint32_t self, const S
 struct BlockKahan {
   float s = 0.f, c = 0.f, p = 0.f;
   __device__ void add(float t) { p += t; }
+  __device__ void add_mul(float a, float b) { p = __builtin_fmaf(a, b, p); }
   __device__ void flush() {
     const float y = p - c;
     const float tt = s + y;
@@ -1373,32 +1440,37 @@ __global__ __launch_bounds__(256) void k_knn_finish(
     double facc;
     if constexpr (FP32C) {
       BlockKahan acc;
-      auto kahan = [&](float q) { acc.add(kern_Ff(kern, __builtin_fminf(q, 1.f))); };
-      auto flush = [&]() { acc.flush(); };
-      // displacements from the fixed-point positions (the minimum image,
-      // exact to L / 2^32) while every neighbour lies within half a box,
-      // scaled straight to units of h; otherwise from the FP64 positions.
-      // Square roots in hardware (within 1 ulp): the FP32 form's error
-      // budget holds them.
-      if (fixed_point_ok(h, lx, ly)) {
-        const uint2 me = UXY[t];
-        const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
-        const float cx = centred_step(xq, S->x0, lx) * gx, cy = centred_step(yq, S->y0, ly) * gy;
-        row_visit<8>(row, k, (uint32_t)t, uxys, [&](uint32_t, uint2 pj) {
+      kern_visit(kern, [&](auto kt) {
+        using KT = decltype(kt);
+        // q of the k-th neighbour may round to a few ulp past 1, where F is
+        // of order ulp^3 (no clamp needed)
+        auto kahan = [&](float q) { acc.add(KT::F(q)); };
+        auto flush = [&]() { acc.flush(); };
+        // displacements from the fixed-point positions (the minimum image,
+        // exact to L / 2^32) while every neighbour lies within half a box,
+        // scaled straight to units of h; otherwise from the FP64 positions.
+        // Square roots in hardware (within 1 ulp): the FP32 form's error
+        // budget holds them.
+        if (fixed_point_ok(h, lx, ly)) {
+          const uint2 me = UXY[t];
+          const float gx = (float)(lx * 0x1p-32 / h), gy = (float)(ly * 0x1p-32 / h);
+          const float cx = centred_step(xq, S->x0, lx) * gx, cy = centred_step(yq, S->y0, ly) * gy;
+          row_visit<8>(row, k, (uint32_t)t, uxys, [&](uint32_t, uint2 pj) {
 #pragma clang fp contract(fast)
-          const float ex = __builtin_fmaf((float)(int32_t)(pj.x - me.x), gx, cx);
-          const float ey = __builtin_fmaf((float)(int32_t)(pj.y - me.y), gy, cy);
-          kahan(__builtin_amdgcn_sqrtf(__builtin_fmaf(ey, ey, ex * ex)));
-        }, flush);
-      } else {
-        // q^2 in FP64 before the FP32 square root (see wave_search)
-        const double itau = 1.0 / tau;
-        row_visit<8>(row, k, (uint32_t)t, XY, [&](uint32_t pk, double2 pj) {
-          const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
-          const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
-          kahan(__builtin_amdgcn_sqrtf((float)((ex * ex + ey * ey) * itau)));
-        }, flush);
-      }
+            const float ex = __builtin_fmaf((float)(int32_t)(pj.x - me.x), gx, cx);
+            const float ey = __builtin_fmaf((float)(int32_t)(pj.y - me.y), gy, cy);
+            kahan(__builtin_amdgcn_sqrtf(__builtin_fmaf(ey, ey, ex * ex)));
+          }, flush);
+        } else {
+          // q^2 in FP64 before the FP32 square root (see wave_search)
+          const double itau = 1.0 / tau;
+          row_visit<8>(row, k, (uint32_t)t, ArrSrc<double2>{XY}, [&](uint32_t pk, double2 pj) {
+            const double ex = (xq + (double)img_ix(pk & 15u) * lx) - pj.x;
+            const double ey = (yq + (double)img_iy(pk & 15u) * ly) - pj.y;
+            kahan(__builtin_amdgcn_sqrtf((float)((ex * ex + ey * ey) * itau)));
+          }, flush);
+        }
+      });
       facc = acc.value();
     } else {
       double acc = 0.0;
@@ -1594,61 +1666,66 @@ __global__ __launch_bounds__(256) void k_force(
       ay_.flush();
       ae_.flush();
     };
-    auto pair = [&](float rx, float ry, const RecF& nb) {
+    kern_visit(kern, [&](auto kt) {
+      using KT = decltype(kt);
+      auto pair = [&](float rx, float ry, const RecF& nb) {
 #pragma clang fp contract(fast)
-      const float d2 = rx * rx + ry * ry;
-      // Two particles within one fixed-point step (L / 2^32) can get a zero
-      // displacement here (or one from the FP64 positions of coincident
-      // particles); their pair term vanishes with the distance (DF(q) ~ q
-      // and dot ~ r as r -> 0): the reciprocal square root of at least
-      // FLT_MIN keeps it 0 (q = 0, DF(0) = 0, r = 0) rather than the
-      // 0 * inf of rsq(0).
-      const float dinv = __builtin_amdgcn_rsqf(__builtin_fmaxf(d2, 0x1p-126f));
-      const float q = __builtin_fminf((d2 * dinv) * hinv, 1.f);
-      const float dR = kern_DFf(kern, q);
-      const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
-      const float dot = vx_ * rx + vy_ * ry;
-      // viscosity where the pair approaches (dot < 0): evaluated for every
-      // pair and selected, since the lanes of a wave take both sides anyway
-      // mu = dot hAB / (r^2 + eta2), pi = mu (beta mu - alpha cAB) / rhoAB,
-      // with one reciprocal of (r^2 + eta2) rhoAB
-      const float cAB = cif + nb.c;
-      const float rhoAB = rhoif + nb.rho;
-      const float hAB = hif + nb.h;
-      const float a_ = dot * hAB;
-      const float inv = __builtin_amdgcn_rcpf((d2 + eta2) * rhoAB);
-      const float muAB = a_ * inv * rhoAB;
-      const float pv = __builtin_fmaf(beta, muAB, -alpha * cAB) * (a_ * inv);
-      const float piAB = dot < 0.f ? pv : 0.f;
-      const float w = (piAB + Af + nb.B) * dR * dinv;
-      ax_.add(rx * w);
-      ay_.add(ry * w);
-      ae_.add(dot * dR);
-    };
-    // 4 records (32 B each) in flight per lane: with most of them in the LDS
-    // window, the registers of more buy less than a fifth wave per SIMD.  A
-    // wave whose queries all take fixed-point displacements (the usual case)
-    // runs a loop without the FP64 form, so every record loads whole.
-    if (__all(fixed)) {
-      row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t, const RecF& nb) {
-        pair(__builtin_fmaf((float)(int32_t)(nb.ux - me.ux), fsx, csx),
-             __builtin_fmaf((float)(int32_t)(nb.uy - me.uy), fsy, csy), nb);
-      }, flush);
-    } else {
-      row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t pk, const RecF& nb) {
-        float rx, ry;
-        if (fixed) {
-          rx = __builtin_fmaf((float)(int32_t)(nb.ux - me.ux), fsx, csx);
-          ry = __builtin_fmaf((float)(int32_t)(nb.uy - me.uy), fsy, csy);
-        } else {
-          const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
-          const double2 pj = XY[pk >> 4];
-          rx = (float)((pj.x - ox) - xi);
-          ry = (float)((pj.y - oy) - yi);
-        }
-        pair(rx, ry, nb);
-      }, flush);
-    }
+        const float d2 = rx * rx + ry * ry;
+        // Two particles within one fixed-point step (L / 2^32) can get a zero
+        // displacement here (or one from the FP64 positions of coincident
+        // particles); their pair term vanishes with the distance (DF(q) ~ q
+        // and dot ~ r as r -> 0): the reciprocal square root of at least
+        // FLT_MIN keeps it 0 (q = 0, DF(0) = 0, r = 0) rather than the
+        // 0 * inf of rsq(0).
+        const float dinv = __builtin_amdgcn_rsqf(__builtin_fmaxf(d2, 0x1p-126f));
+        // q of the k-th neighbour may round to a few ulp past 1, where DF is
+        // of order ulp^2 (no clamp needed)
+        const float q = (d2 * dinv) * hinv;
+        const float dR = KT::DF(q);
+        const float vx_ = nb.vpx - vpxi, vy_ = nb.vpy - vpyi;
+        const float dot = vx_ * rx + vy_ * ry;
+        // viscosity where the pair approaches (dot < 0), for every pair from
+        // min(dot, 0) (mu and pi vanish otherwise; the lanes of a wave take
+        // both sides anyway): mu = dot hAB / (r^2 + eta2),
+        // pi = mu (beta mu - alpha cAB) / rhoAB, with one reciprocal of
+        // (r^2 + eta2) rhoAB
+        const float cAB = cif + nb.c;
+        const float rhoAB = rhoif + nb.rho;
+        const float hAB = hif + nb.h;
+        const float a_ = __builtin_fminf(dot, 0.f) * hAB;
+        const float inv = __builtin_amdgcn_rcpf((d2 + eta2) * rhoAB);
+        const float muAB = a_ * inv * rhoAB;
+        const float piAB = __builtin_fmaf(beta, muAB, -alpha * cAB) * (a_ * inv);
+        const float w = (piAB + Af + nb.B) * dR * dinv;
+        ax_.add_mul(rx, w);
+        ay_.add_mul(ry, w);
+        ae_.add_mul(dot, dR);
+      };
+      // 4 records (32 B each) in flight per lane: with most of them in the LDS
+      // window, the registers of more buy less than a fifth wave per SIMD.  A
+      // wave whose queries all take fixed-point displacements (the usual case)
+      // runs a loop without the FP64 form, so every record loads whole.
+      if (__all(fixed)) {
+        row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t, const RecF& nb) {
+          pair(__builtin_fmaf((float)(int32_t)(nb.ux - me.ux), fsx, csx),
+               __builtin_fmaf((float)(int32_t)(nb.uy - me.uy), fsy, csy), nb);
+        }, flush);
+      } else {
+        row_visit<4, true>(row, k, (uint32_t)t, wsrc, [&](uint32_t pk, const RecF& nb) {
+          float rx, ry;
+          if (fixed) {
+            rx = __builtin_fmaf((float)(int32_t)(nb.ux - me.ux), fsx, csx);
+            ry = __builtin_fmaf((float)(int32_t)(nb.uy - me.uy), fsy, csy);
+          } else {
+            const double ox = (double)img_ix(pk & 15u) * lx, oy = (double)img_iy(pk & 15u) * ly;
+            const double2 pj = XY[pk >> 4];
+            rx = (float)((pj.x - ox) - xi);
+            ry = (float)((pj.y - oy) - yi);
+          }
+          pair(rx, ry, nb);
+        }, flush);
+      }
+    });
     sx = ax_.value();
     sy = ay_.value();
     se = ae_.value();
