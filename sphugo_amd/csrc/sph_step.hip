@@ -1629,10 +1629,12 @@ __global__ __launch_bounds__(256) void k_force(
         const double dR = kern_DF_fma(kern, q);
         double piAB = 0.0;
         if (dot < 0) {
-          const double cAB = cih + nb.c;
-          const double hAB = hih + nb.h;
-          const double muAB = dot * hAB * rcp64(r2 + eta2);
-          piAB = (-alpha * cAB * muAB + beta * muAB * muAB) * rcp64(rhoih + nb.rho);
+          // pi = mu (beta mu - alpha cAB) / rhoAB with one reciprocal:
+          // t = mu / rhoAB = dot hAB / ((r^2 + eta2) rhoAB)
+          const double rhoAB = rhoih + nb.rho;
+          const double t = dot * (hih + nb.h) * rcp64((r2 + eta2) * rhoAB);
+          const double muAB = t * rhoAB;
+          piAB = (beta * muAB - alpha * (cih + nb.c)) * t;
         }
         const double w = (piAB + A + nb.B) * dR * rinv;
         sx += rABx * w;
