@@ -245,11 +245,18 @@ int sph_set_graph(sph_sim* s, int32_t on);
  * neighbours are summed (results within the precision form's contract). */
 int sph_autotune(sph_sim* s, double* best_ms, double* best_cells_per_particle);
 /* The grid and walk shape in use: cells per particle, bracket half-width
- * around each query's last h, split threshold, first lane group, sub-tiles. */
+ * around each query's last h, split threshold, first lane group, sub-tiles.
+ * The bracket adapts per simulation after every pass (widened by a quarter
+ * when more than 1/128 of the queries fell outside it because h moved,
+ * narrowed by a fifth when more than 1/128 overflow its annulus, within
+ * 0.0125..0.1, and otherwise moved a tenth back towards the value set here);
+ * the value returned is simulation 0's, where its next pass starts (waits
+ * for the handle's work). */
 int sph_get_tuning(sph_sim* s, double out[5]);
 /* Set them (same order and ranges: a grid the autotuner may choose, bracket
  * 0.005..0.1, split 8..64, group 16/32/64, sub-tiles 4/8); SPH_EINVAL
- * otherwise.  Any time between steps. */
+ * otherwise.  The bracket restarts from the given value in every
+ * simulation.  Any time between steps. */
 int sph_set_tuning(sph_sim* s, const double in[5]);
 
 /* ---- serve loop: continuous batching (no reference counterpart) ----

@@ -133,6 +133,8 @@ void sph::fill_simdev(SimDev& d, const sph_params* p, int64_t n, int lg) {
   d.cfac = p->gamma * (p->gamma - 1);
   d.dt_fixed = p->dt_half;
   d.dt_half = p->dt_half > 0 ? p->dt_half : 0.0;
+  d.walk_delta = kWalkDefault.delta;
+  d.walk_delta0 = kWalkDefault.delta;
   d.r_init = 1.2 * std::sqrt(p->k * d.lx * d.ly / (M_PI * (double)n));
 }
 
@@ -199,6 +201,7 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
                      np * (int64_t)(h->prec == SPH_PREC_FP32C ? sizeof(RecF) : sizeof(RecD)))) ||
       (rc = h->alloc(&pb.partials, (int64_t)nitems * kNumRed)) ||
       (rc = h->alloc(&pb.stats, kStatSlots)) || (rc = h->alloc(&pb.err, 1 + pb.nsims)) ||
+      (rc = h->alloc(&pb.walk_miss, (int64_t)pb.nsims * kMissSlots * kMissStride)) ||
       (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
       (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, (np + 63) / 64 * 64 * kAnn)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
@@ -325,6 +328,8 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
         return bail(fail(SPH_EDEVICE, "memset failed"));
   }
   if (hipMemsetAsync(pb.stats, 0, kStatSlots * sizeof(KnnStats), h->st) != hipSuccess ||
+      hipMemsetAsync(pb.walk_miss, 0, (int64_t)pb.nsims * kMissSlots * kMissStride * 4, h->st) !=
+          hipSuccess ||
       hipMemsetAsync(pb.err, 0, (1 + pb.nsims) * sizeof(int), h->st) != hipSuccess ||
       hipMemsetAsync(pb.nbr, 0, nbr_entries(np, pb.kmax) * sizeof(uint32_t), h->st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "memset failed"));
@@ -572,8 +577,10 @@ static bool valid_knobs(const WalkKnobs& k) {
 
 extern "C" int sph_get_tuning(sph_sim* s, double out[5]) {
   if (!s || !out) return fail(SPH_EINVAL, "null argument");
+  int rc = fetch_sims(s);  // the bracket adapted by the passes so far
+  if (rc) return rc;
   out[0] = s->grid_cpp;
-  out[1] = s->pb.walk.delta;
+  out[1] = s->hsims[0].walk_delta;
   out[2] = s->pb.walk.split;
   out[3] = s->pb.walk.group;
   out[4] = s->pb.walk.nsub;
@@ -597,6 +604,7 @@ extern "C" int sph_set_tuning(sph_sim* s, const double in[5]) {
   s->drop_graphs();
   layout_cells(s, in[0]);
   s->pb.walk = k;
+  for (SimDev& d : s->hsims) d.walk_delta = d.walk_delta0 = k.delta;  // where every bracket restarts
   HIPCHK(hipMemcpyAsync(s->pb.sims, s->hsims.data(), s->hsims.size() * sizeof(SimDev),
                         hipMemcpyHostToDevice, s->st));
   HIPCHK(hipStreamSynchronize(s->st));
@@ -908,6 +916,8 @@ void take_device_scalars(SimDev& d, const SimDev& from) {
   d.cfl_min_bits = from.cfl_min_bits;
   for (int r = 0; r < kNumRed; ++r) d.red[r] = from.red[r];
   d.last_speed = from.last_speed;
+  d.walk_delta = from.walk_delta;
+  d.walk_delta0 = from.walk_delta0;
 }
 }  // namespace
 
@@ -1010,6 +1020,9 @@ extern "C" int sph_checkpoint_load(const void* buf, int64_t bytes, sph_sim** out
       return fail(SPH_EINVAL, "corrupt checkpoint (simulation record does not match its parameters)");
     if (!std::isfinite(sd.dt_half) || sd.dt_half < 0 || !std::isfinite(sd.dt_last))
       return fail(SPH_EINVAL, "corrupt checkpoint (half step)");
+    if (!(sd.walk_delta >= 0.005 && sd.walk_delta <= kDeltaMax && sd.walk_delta0 >= 0.005 &&
+          sd.walk_delta0 <= kDeltaMax))
+      return fail(SPH_EINVAL, "corrupt checkpoint (kNN bracket)");
   }
   // aligned host copies of what sph_create takes; the rest goes straight up
   std::vector<double> init(5 * n);

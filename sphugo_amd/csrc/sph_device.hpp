@@ -10,6 +10,11 @@ constexpr int kItem = 256;        // particles per work item (one block)
 constexpr int kMaxK = 64;         // neighbours per particle
 constexpr int kCandCap = 256;     // kNN candidates held per wave
 constexpr int kNumRed = 5;        // sum e, sum rho, E, Px, Py
+// the walk's miss counts of a pass in progress, per simulation: kMissSlots
+// copies, one 128-byte line each ([0]: h moved past the bracket, [1]:
+// annulus overflow), blocks adding to copy blockIdx % kMissSlots
+constexpr int kMissSlots = 16;
+constexpr int kMissStride = 32;   // u32 per copy (one cache line)
 constexpr int kAnn = 16;          // kNN annulus entries kept per query
 
 // Go folds 6*40/(Pi*7), 4*7/(Pi*4), 8*7/(Pi*4) exactly and rounds once
@@ -57,7 +62,34 @@ struct SimDev {
   unsigned long long cfl_min_bits;  // atomicMin over non-negative doubles
   double red[kNumRed];  // reductions of the last pass
   double last_speed;    // |v| of the particle stored last
+  // kNN walk bracket half-width of this simulation, adapted after each pass
+  // from the queries its walk could not resolve (k_reduce): misses because
+  // h moved past the bracket widen it, annulus overflows narrow it, and it
+  // returns towards walk_delta0 (the configured value) when misses are rare
+  double walk_delta;
+  double walk_delta0;
 };
+
+// Bracket adaptation after a pass with n queries, from its walk's misses
+// because h moved past the bracket (fewer than k within it, or k within its
+// inner radius) and its annulus overflows: a miss rate above 1/128
+// of either kind moves delta by a quarter (towards the kind that dominates),
+// not past [kDeltaMin, kDeltaMax]; otherwise it moves a tenth back towards
+// the configured d0 (so a burst of motion widens it for as long as it
+// lasts).  Neighbour sets and h do not depend on delta (every miss is
+// searched exactly); the share of queries the walk resolves does, and
+// outside the bitwise form so does the order in which a row's neighbours
+// are summed.  The counts are integers, so the adaptation is deterministic.
+constexpr double kDeltaMin = 0.0125, kDeltaMax = 0.1;
+__host__ __device__ inline double adapt_delta(double d, double d0, uint32_t move, uint32_t wide,
+                                              int32_t n) {
+  const double thr = n / 128.0;
+  if (move > thr && move >= wide) return d * 1.25 < kDeltaMax ? d * 1.25 : (d > kDeltaMax ? d : kDeltaMax);
+  if (wide > thr) return d * 0.8 > kDeltaMin ? d * 0.8 : (d < kDeltaMin ? d : kDeltaMin);
+  if (d == d0) return d;
+  const double t = d > d0 ? d * 0.9 : d * 1.1;
+  return (d > d0) == (t > d0) ? t : d0;
+}
 
 // Grid box of a simulation's open axes around the range [lo, hi] of its pass
 // positions per axis (the reference searches no periodic images on such an

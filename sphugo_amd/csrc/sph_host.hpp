@@ -62,6 +62,7 @@ struct PassBuffers {
   void* rec;            // np force records (RecD or RecF)
   double* partials;     // nitems * kNumRed
   KnnStats* stats;      // kStatSlots copies
+  uint32_t* walk_miss;  // per simulation: the walk's miss counts (kMissSlots lines)
   uint2* retry;           // kNN retry list (np entries)
   uint32_t* retry_count;
   uint32_t* knn_cnt;       // per query: certain | annulus << 8, or ~0 (retry)

@@ -795,7 +795,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
     uint32_t* __restrict__ nbr, int kstride, double* __restrict__ RHO,
     double* __restrict__ C, void* __restrict__ recp, KnnStats* __restrict__ stats,
     uint2* __restrict__ retry, uint32_t* __restrict__ retry_count,
-    uint32_t* __restrict__ cnt, uint32_t* __restrict__ ann, WalkKnobs kn) {
+    uint32_t* __restrict__ cnt, uint32_t* __restrict__ ann, WalkKnobs kn,
+    uint32_t* __restrict__ walk_miss) {
   constexpr int kSub = KSUB;       // sub-tiles of a wave
   constexpr int kSubW = 64 / KSUB; // consecutive queries per sub-tile
   __shared__ TileShared<KSUB> wsh[4];
@@ -821,7 +822,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   bool resolved = false;
   double tau = 0.0;
   if constexpr (!BITWISE) {
-    const double Rlo = hp * (1 - kn.delta), Rhi = hp * (1 + kn.delta);
+    const double delta = S->walk_delta;
+    const double Rlo = hp * (1 - delta), Rhi = hp * (1 + delta);
     const double Rlo2 = Rlo * Rlo, Rhi2 = Rhi * Rhi;
     const double Rm = Rhi * (1 + 1e-12) + 1e-15 * (lx + ly);
     int c_lo = 0, c_an = 0;
@@ -1147,6 +1149,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void k
   const unsigned long long nlo = __popcll(ballot(miss_lo)), nhi = __popcll(ballot(miss_hi));
   const unsigned long long nann = __popcll(ballot(miss_ann)), nsplit = __popcll(ballot(split_fb));
   if (lane == 0) {
+    // the simulation's bracket misses (k_reduce adapts its delta)
+    uint32_t* ms = walk_miss + ((int64_t)it.sim * kMissSlots + (blockIdx.x & (kMissSlots - 1))) *
+                                   kMissStride;
+    if (nlo + nhi) atomicAdd(ms, (uint32_t)(nlo + nhi));
+    if (nann) atomicAdd(ms + 1, (uint32_t)nann);
     KnnStats* sp = stat_slot(stats);
     stat_add(&sp->retries, nfall);
     stat_add(&sp->miss_lo, nlo);
@@ -1818,7 +1825,8 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
                                                 const double* __restrict__ partials,
                                                 const int* __restrict__ item_first,
                                                 const int* __restrict__ item_count, Fields f,
-                                                int mode) {
+                                                int mode, uint32_t* __restrict__ walk_miss,
+                                                int walked) {
   __shared__ double sh[kNumRed][256];
   SimDev* S = sims + blockIdx.x;
   if (S->n == 0) return;  // unused slot
@@ -1850,6 +1858,15 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
     const double vx = f.vx[last], vy = f.vy[last];
     S->last_speed = sqrt(vx * vx + vy * vy);
     if (mode == kForceKick) S->dt_last = S->dt_half;
+    uint32_t move = 0u, wide = 0u;
+    uint32_t* wm = walk_miss + (int64_t)blockIdx.x * kMissSlots * kMissStride;
+    for (int q = 0; q < kMissSlots; ++q) {
+      move += wm[q * kMissStride];
+      wide += wm[q * kMissStride + 1];
+      wm[q * kMissStride] = 0u;
+      wm[q * kMissStride + 1] = 0u;
+    }
+    if (walked) S->walk_delta = adapt_delta(S->walk_delta, S->walk_delta0, move, wide, S->n);
     const double m = __longlong_as_double((long long)S->cfl_min_bits);
     S->dt_half = S->use_cfl ? 0.5 * (S->cfl * m) : S->dt_fixed;
     S->cfl_min_bits = (unsigned long long)__double_as_longlong(INFINITY);
@@ -1901,7 +1918,8 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   hipLaunchKernelGGL((k_knn<false, F32, NSUB>), grd, dim3(256), 0, st, b.items, b.nitems,      \
                      b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h, b.fnew.ep,                      \
                      b.fnew.vpx, b.fnew.vpy, b.fnew.id, b.nbr, b.kmax, b.fnew.rho, b.fnew.c,    \
-                     b.rec, b.stats, b.retry, b.retry_count, b.knn_cnt, b.knn_ann, b.walk)
+                     b.rec, b.stats, b.retry, b.retry_count, b.knn_cnt, b.knn_ann, b.walk,      \
+                     b.walk_miss)
 #define SPH_FB(BW, F32)                                                                       \
   hipLaunchKernelGGL((k_knn_fallback<BW, F32>), dim3(fb_grid), dim3(256), 0, st, b.items,     \
                      b.nitems, b.sims, b.xyp, b.wrap, b.cstart, b.fnew.h,                       \
@@ -1950,7 +1968,8 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   mark(4, 1);
   mark(5, 0);
   hipLaunchKernelGGL(k_reduce, dim3(b.nsims), dim3(256), 0, st, b.sims, b.partials,
-                     b.red_first, b.red_count, b.fnew, force_mode);
+                     b.red_first, b.red_count, b.fnew, force_mode, b.walk_miss,
+                     (!b.bitwise && b.have_h) ? 1 : 0);
   mark(5, 1);
   return (int)hipGetLastError();
 }

@@ -160,3 +160,33 @@ def test_set_tuning_rejects():
                 dict(cells_per_particle=3.0)):
         with pytest.raises(L.SphError):
             L.set_tuning(a, **bad)
+
+
+def test_bracket_adapts_to_motion():
+    """A blast whose h moves faster than the default bracket: the bracket
+    widens after passes where more than 1/128 of the queries missed, the
+    share the walk leaves to the exact search falls, and a pass with the
+    adapted bracket still finds the oracle's neighbour sets and forces."""
+    import oracle as O
+    from test_gpu_parity import _check_forces, _check_knn, state_pass_vs_oracle
+    st = ics.blast(20000, seed=9, k=32)
+    a = L.Sim(_p(st, k=32), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    moved = []
+    for _ in range(8):
+        c0 = a.knn_counters()
+        a.step(1)
+        c1 = a.knn_counters()
+        moved.append((c1["miss_lo"] + c1["miss_hi"]) - (c0["miss_lo"] + c0["miss_hi"]))
+    d = L.get_tuning(a)["delta"]
+    assert 0.0125 <= d <= 0.1
+    if max(moved[:3]) > st["n"] / 128:
+        assert d > 0.025, (moved, d)
+        assert moved[-1] < max(moved[:3]), (moved, d)
+    op = O.default_params(knn_mode=O.KNN_EXACT, use_cfl=1, k=32, mass=st["mass"], gamma=5 / 3,
+                          cfl=0.3, hor=(0, 1), vert=(0, 1), visc_alpha=1.0, visc_beta=2.0)
+    o, _ = state_pass_vs_oracle(a, op)
+    _check_knn(a, o)
+    _check_forces(a, o)
+    # a bracket set by the caller restarts every simulation from that value
+    L.set_tuning(a, delta=0.02)
+    assert L.get_tuning(a)["delta"] == 0.02
