@@ -1820,7 +1820,7 @@ __global__ __launch_bounds__(256) void k_force(
 }
 
 // One block per simulation: totals over its items in a fixed order, the
-// next CFL half step, the TotalMomentum quirk value.
+// next CFL half step, the TotalMomentum quirk value and the next bracket.
 __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
                                                 const double* __restrict__ partials,
                                                 const int* __restrict__ item_first,
@@ -1828,11 +1828,23 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
                                                 int mode, uint32_t* __restrict__ walk_miss,
                                                 int walked) {
   __shared__ double sh[kNumRed][256];
+  __shared__ uint32_t miss[2][kMissSlots];
   SimDev* S = sims + blockIdx.x;
   if (S->n == 0) return;  // unused slot
   if (mode < 0) mode = S->fmode;
   const int tid = threadIdx.x;
   const int i0 = item_first[blockIdx.x], ni = item_count[blockIdx.x];
+  // the walk's miss counts of this pass (copy q by thread q), read and
+  // cleared for the next pass while the sums are under way
+  if (tid < kMissSlots) {
+    uint32_t* wm = walk_miss + ((int64_t)blockIdx.x * kMissSlots + tid) * kMissStride;
+    miss[0][tid] = wm[0];
+    miss[1][tid] = wm[1];
+    wm[0] = 0u;
+    wm[1] = 0u;
+  }
+  const int64_t last = S->off + S->n - 1;
+  const double vx_last = tid == 0 ? f.vx[last] : 0.0, vy_last = tid == 0 ? f.vy[last] : 0.0;
   double acc[kNumRed];
 #pragma unroll
   for (int r = 0; r < kNumRed; ++r) acc[r] = 0.0;
@@ -1854,17 +1866,12 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
   if (tid == 0) {
 #pragma unroll
     for (int r = 0; r < kNumRed; ++r) S->red[r] = sh[r][0];
-    const int64_t last = S->off + S->n - 1;
-    const double vx = f.vx[last], vy = f.vy[last];
-    S->last_speed = sqrt(vx * vx + vy * vy);
+    S->last_speed = sqrt(vx_last * vx_last + vy_last * vy_last);
     if (mode == kForceKick) S->dt_last = S->dt_half;
     uint32_t move = 0u, wide = 0u;
-    uint32_t* wm = walk_miss + (int64_t)blockIdx.x * kMissSlots * kMissStride;
     for (int q = 0; q < kMissSlots; ++q) {
-      move += wm[q * kMissStride];
-      wide += wm[q * kMissStride + 1];
-      wm[q * kMissStride] = 0u;
-      wm[q * kMissStride + 1] = 0u;
+      move += miss[0][q];
+      wide += miss[1][q];
     }
     if (walked) S->walk_delta = adapt_delta(S->walk_delta, S->walk_delta0, move, wide, S->n);
     const double m = __longlong_as_double((long long)S->cfl_min_bits);
