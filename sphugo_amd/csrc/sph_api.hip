@@ -202,6 +202,7 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
       (rc = h->alloc(&pb.partials, (int64_t)nitems * kNumRed)) ||
       (rc = h->alloc(&pb.stats, kStatSlots)) || (rc = h->alloc(&pb.err, 1 + pb.nsims)) ||
       (rc = h->alloc(&pb.walk_miss, (int64_t)pb.nsims * kMissSlots * kMissStride)) ||
+      (rc = h->alloc(&pb.red_part, (int64_t)pb.nsims * kRedSplit * kNumRed)) ||
       (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
       (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, (np + 63) / 64 * 64 * kAnn)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
@@ -277,6 +278,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
       h->hitems.push_back(it);
     }
     red_count[s] = (int)h->hitems.size() - red_first[s];
+    h->pb.red_parts_max = std::max(h->pb.red_parts_max, red_parts(red_count[s]));
     off += ns[s];
   }
   // cell storage for the finest grid the autotuner may choose

@@ -10,6 +10,13 @@ constexpr int kItem = 256;        // particles per work item (one block)
 constexpr int kMaxK = 64;         // neighbours per particle
 constexpr int kCandCap = 256;     // kNN candidates held per wave
 constexpr int kNumRed = 5;        // sum e, sum rho, E, Px, Py
+constexpr int kRedSplit = 16;     // first-stage reduction blocks per simulation
+// first-stage reduction blocks of a simulation with nitems work items (one
+// per 1024 items, at most kRedSplit; 1: its totals are summed in one stage)
+__host__ __device__ inline int red_parts(int nitems) {
+  const int p = (nitems + 1023) / 1024;
+  return p < 1 ? 1 : (p > kRedSplit ? kRedSplit : p);
+}
 // the walk's miss counts of a pass in progress, per simulation: kMissSlots
 // copies, one 128-byte line each ([0]: h moved past the bracket, [1]:
 // annulus overflow), blocks adding to copy blockIdx % kMissSlots

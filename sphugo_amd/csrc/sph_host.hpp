@@ -63,6 +63,8 @@ struct PassBuffers {
   double* partials;     // nitems * kNumRed
   KnnStats* stats;      // kStatSlots copies
   uint32_t* walk_miss;  // per simulation: the walk's miss counts (kMissSlots lines)
+  double* red_part;     // per simulation: kRedSplit first-stage sums (kNumRed each)
+  int red_parts_max;    // the largest red_parts over the simulations (1: one stage)
   uint2* retry;           // kNN retry list (np entries)
   uint32_t* retry_count;
   uint32_t* knn_cnt;       // per query: certain | annulus << 8, or ~0 (retry)

@@ -194,8 +194,11 @@ int upload_items(sph_server* s) {
     if (h.first < hi) push_items(s->h_items_all, -1, h.first, std::min(h.second, hi - h.first));
   PassBuffers& pb = s->core.pb;
   pb.any_open = 0;
-  for (int q = 0; q < s->max_jobs; ++q)
+  pb.red_parts_max = 1;
+  for (int q = 0; q < s->max_jobs; ++q) {
     if (s->slots[q] && s->hsim[q].open) pb.any_open = 1;
+    pb.red_parts_max = std::max(pb.red_parts_max, red_parts(s->h_rc[q]));
+  }
   pb.nitems = (int)s->h_items.size();
   pb.nitems_all = (int)s->h_items_all.size();
   pb.np = hi;

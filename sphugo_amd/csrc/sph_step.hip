@@ -1819,10 +1819,56 @@ __global__ __launch_bounds__(256) void k_force(
   }
 }
 
-// One block per simulation: totals over its items in a fixed order, the
+// Totals over a simulation's work items in a fixed order.  A simulation of
+// more than 1024 items is summed in two stages: red_parts blocks each sum a
+// contiguous share of its items (k_reduce_part), then k_reduce adds those
+// sums in block order; a smaller one is summed by k_reduce alone.  The order
+// of every addition depends only on the simulation's own item count, so
+// totals are reproducible and the same alone, batched and served.
+__device__ inline void block_sum(double (*sh)[256], double (&acc)[kNumRed]) {
+  const int tid = threadIdx.x;
+#pragma unroll
+  for (int r = 0; r < kNumRed; ++r) sh[r][tid] = acc[r];
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (tid < o)
+#pragma unroll
+      for (int r = 0; r < kNumRed; ++r) sh[r][tid] += sh[r][tid + o];
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_part(const SimDev* __restrict__ sims,
+                                                     const double* __restrict__ partials,
+                                                     const int* __restrict__ item_first,
+                                                     const int* __restrict__ item_count,
+                                                     double* __restrict__ red_part,
+                                                     int parts_max) {
+  __shared__ double sh[kNumRed][256];
+  const int sim = blockIdx.x / parts_max, part = blockIdx.x % parts_max;
+  if (sims[sim].n == 0) return;  // unused slot
+  const int i0 = item_first[sim], ni = item_count[sim];
+  const int nparts = red_parts(ni);
+  if (nparts == 1 || part >= nparts) return;
+  const int chunk = (ni + nparts - 1) / nparts;
+  const int ia = part * chunk, ib = ia + chunk < ni ? ia + chunk : ni;
+  double acc[kNumRed];
+#pragma unroll
+  for (int r = 0; r < kNumRed; ++r) acc[r] = 0.0;
+#pragma unroll 4
+  for (int i = ia + (int)threadIdx.x; i < ib; i += 256)
+#pragma unroll
+    for (int r = 0; r < kNumRed; ++r) acc[r] += partials[(int64_t)(i0 + i) * kNumRed + r];
+  block_sum(sh, acc);
+  if (threadIdx.x < kNumRed)
+    red_part[((int64_t)sim * kRedSplit + part) * kNumRed + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+// One block per simulation: the totals from the first stage's sums, the
 // next CFL half step, the TotalMomentum quirk value and the next bracket.
 __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
                                                 const double* __restrict__ partials,
+                                                const double* __restrict__ red_part,
                                                 const int* __restrict__ item_first,
                                                 const int* __restrict__ item_count, Fields f,
                                                 int mode, uint32_t* __restrict__ walk_miss,
@@ -1833,9 +1879,8 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
   if (S->n == 0) return;  // unused slot
   if (mode < 0) mode = S->fmode;
   const int tid = threadIdx.x;
-  const int i0 = item_first[blockIdx.x], ni = item_count[blockIdx.x];
   // the walk's miss counts of this pass (copy q by thread q), read and
-  // cleared for the next pass while the sums are under way
+  // cleared for the next pass
   if (tid < kMissSlots) {
     uint32_t* wm = walk_miss + ((int64_t)blockIdx.x * kMissSlots + tid) * kMissStride;
     miss[0][tid] = wm[0];
@@ -1845,24 +1890,24 @@ __global__ __launch_bounds__(256) void k_reduce(SimDev* __restrict__ sims,
   }
   const int64_t last = S->off + S->n - 1;
   const double vx_last = tid == 0 ? f.vx[last] : 0.0, vy_last = tid == 0 ? f.vy[last] : 0.0;
+  const int i0 = item_first[blockIdx.x], ni = item_count[blockIdx.x];
+  const int nparts = red_parts(ni);
   double acc[kNumRed];
 #pragma unroll
   for (int r = 0; r < kNumRed; ++r) acc[r] = 0.0;
-  // unrolled so that many items' loads are in flight (one block reduces a
-  // whole simulation); the additions keep their order
+  if (nparts == 1) {
+    // unrolled so that many items' loads are in flight; the additions keep
+    // their order
 #pragma unroll 8
-  for (int i = tid; i < ni; i += 256)
+    for (int i = tid; i < ni; i += 256)
 #pragma unroll
-    for (int r = 0; r < kNumRed; ++r) acc[r] += partials[(int64_t)(i0 + i) * kNumRed + r];
+      for (int r = 0; r < kNumRed; ++r) acc[r] += partials[(int64_t)(i0 + i) * kNumRed + r];
+  } else if (tid < nparts) {
 #pragma unroll
-  for (int r = 0; r < kNumRed; ++r) sh[r][tid] = acc[r];
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (tid < o)
-#pragma unroll
-      for (int r = 0; r < kNumRed; ++r) sh[r][tid] += sh[r][tid + o];
-    __syncthreads();
+    for (int r = 0; r < kNumRed; ++r)
+      acc[r] = red_part[((int64_t)blockIdx.x * kRedSplit + tid) * kNumRed + r];
   }
+  block_sum(sh, acc);
   if (tid == 0) {
 #pragma unroll
     for (int r = 0; r < kNumRed; ++r) S->red[r] = sh[r][0];
@@ -1974,7 +2019,10 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
 #undef SPH_FORCE
   mark(4, 1);
   mark(5, 0);
-  hipLaunchKernelGGL(k_reduce, dim3(b.nsims), dim3(256), 0, st, b.sims, b.partials,
+  if (b.red_parts_max > 1)
+    hipLaunchKernelGGL(k_reduce_part, dim3(b.nsims * b.red_parts_max), dim3(256), 0, st, b.sims,
+                       b.partials, b.red_first, b.red_count, b.red_part, b.red_parts_max);
+  hipLaunchKernelGGL(k_reduce, dim3(b.nsims), dim3(256), 0, st, b.sims, b.partials, b.red_part,
                      b.red_first, b.red_count, b.fnew, force_mode, b.walk_miss,
                      (!b.bitwise && b.have_h) ? 1 : 0);
   mark(5, 1);
