@@ -249,7 +249,8 @@ int sph_autotune(sph_sim* s, double* best_ms, double* best_cells_per_particle);
  * The bracket adapts per simulation after every pass (widened by a quarter
  * when more than 1/128 of the queries fell outside it because h moved,
  * narrowed by a fifth when more than 1/128 overflow its annulus, within
- * 0.0125..0.1, and otherwise moved a tenth back towards the value set here);
+ * 0.0125..0.1, moved a tenth back towards the value set here once both
+ * kinds stay below 1/512, and held in between);
  * the value returned is simulation 0's, where its next pass starts (waits
  * for the handle's work). */
 int sph_get_tuning(sph_sim* s, double out[5]);

@@ -81,9 +81,10 @@ struct SimDev {
 // because h moved past the bracket (fewer than k within it, or k within its
 // inner radius) and its annulus overflows: a miss rate above 1/128
 // of either kind moves delta by a quarter (towards the kind that dominates),
-// not past [kDeltaMin, kDeltaMax]; otherwise it moves a tenth back towards
-// the configured d0 (so a burst of motion widens it for as long as it
-// lasts).  Neighbour sets and h do not depend on delta (every miss is
+// not past [kDeltaMin, kDeltaMax]; once both stay below a quarter of that
+// rate it moves a tenth back towards the configured d0, and in between it
+// holds (so a burst of motion widens it for as long as it lasts, and it
+// does not settle at the edge of the widening threshold).  Neighbour sets and h do not depend on delta (every miss is
 // searched exactly); the share of queries the walk resolves does, and
 // outside the bitwise form so does the order in which a row's neighbours
 // are summed.  The counts are integers, so the adaptation is deterministic.
@@ -93,7 +94,7 @@ __host__ __device__ inline double adapt_delta(double d, double d0, uint32_t move
   const double thr = n / 128.0;
   if (move > thr && move >= wide) return d * 1.25 < kDeltaMax ? d * 1.25 : (d > kDeltaMax ? d : kDeltaMax);
   if (wide > thr) return d * 0.8 > kDeltaMin ? d * 0.8 : (d < kDeltaMin ? d : kDeltaMin);
-  if (d == d0) return d;
+  if (d == d0 || move > thr / 4 || wide > thr / 4) return d;
   const double t = d > d0 ? d * 0.9 : d * 1.1;
   return (d > d0) == (t > d0) ? t : d0;
 }
