@@ -24,7 +24,8 @@
 //                   with kick, drift 2, periodic wrap and reflection
 //                   (sim/sph.go:327-401,122-193), partial sums and the CFL
 //                   minimum
-//   k_reduce        per-simulation FP64 totals and the next CFL half step
+//   k_reduce_part,  per-simulation FP64 totals in a fixed order, the next
+//   k_reduce        CFL half step and the next kNN walk bracket
 //
 // All FP64 arithmetic on the parity path is evaluated in the reference's
 // association order; the library is compiled with -ffp-contract=off so no
