@@ -14,7 +14,10 @@ reference (tree build + 9-image tree kNN), timed on this host.
 
 The serving workload (`serve` in the line) pushes 1000 jobs with Poisson
 arrivals through the continuous-batching server and reports throughput and
-p50/p99 submit-to-done latency.
+p50/p99 submit-to-done latency.  `speed_test` times the reference's own
+harness (examples/speed-test/speed-test.go: 100,000 uniform particles on
+MakeConfig's defaults, 20 Step() calls) on the GPU, beside one thread of the
+oracle on the same input in the CPU baseline.
 """
 from __future__ import annotations
 
@@ -149,6 +152,42 @@ def run_batch(steps=10, warmup=3):
     return out
 
 
+SPEED_TEST_STEPS = 20  # Step() calls of the reference's harness
+
+
+def speed_test_params(precision=L.PREC_FP64):
+    """MakeConfig() (sim/config-parser.go:131-149: both axes open, no walls,
+    Monaghan, gamma 1.66666, mass 1, k = NN_SIZE = 32, viscosity 0.75 / 1.5 /
+    0.01) with the harness's DeltaTHalf 0.02 and Acceleration (0, 0.2)
+    (examples/speed-test/speed-test.go:25-27)."""
+    return L.default_params(precision=precision, dt_half=0.02, accel=(0.0, 0.2))
+
+
+def run_speed_test():
+    """The reference's timing harness on the GPU: a fresh simulation of its
+    100,000-particle input, SPEED_TEST_STEPS Step() calls from creation (the
+    first with its extra force pass), as frames per second the way the
+    harness prints them, then the steady rate of the next SPEED_TEST_STEPS."""
+    st = ics.speed_test()
+    out = {"workload": "the reference's speed-test harness (examples/speed-test/speed-test.go): "
+                       "100,000 uniform particles, MakeConfig defaults (open axes), "
+                       "DeltaTHalf 0.02, g = (0, 0.2), k = 32",
+           "n_particles": st["n"], "steps": SPEED_TEST_STEPS}
+    for name, prec in (("fp64", L.PREC_FP64), ("fp32c", L.PREC_FP32C)):
+        sim = L.Sim(speed_test_params(prec), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+        sim.synchronize()
+        t0 = time.perf_counter()
+        sim.step(SPEED_TEST_STEPS)  # returns after the device finished
+        t1 = time.perf_counter()
+        sim.step(SPEED_TEST_STEPS)
+        t2 = time.perf_counter()
+        sim.close()
+        out[name] = {"fps": round(SPEED_TEST_STEPS / (t1 - t0), 1),
+                     "steady_particle_steps_per_s": round(st["n"] * SPEED_TEST_STEPS / (t2 - t1), 1)}
+        print(f"# speed test {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
+
+
 def cpu_info():
     model = "unknown"
     try:
@@ -192,6 +231,16 @@ def cpu_baseline(threads_all: int, c3_steps: int = 1):
           file=sys.stderr, flush=True)
     small = ics.clustered(32768, seed=3, k=64)
     v_one, s_one = rate(small, 1, 4)
+    # the reference's speed-test harness input, one thread (the reference's
+    # path is serial): its first Step() calls from creation, as it times them
+    stt = ics.speed_test()
+    ps = O.default_params(k=32, knn_mode=O.KNN_TREE, dt_half=0.02, accel=(0.0, 0.2),
+                          use_cfl=0)
+    s = O.OracleSim(ps, stt["x"], stt["y"], stt["vx"], stt["vy"], stt["e"], threads=1)
+    nst = 3
+    t0 = time.perf_counter()
+    s.step(nst)
+    st_fps = nst / (time.perf_counter() - t0)
     return {
         "value": v_all, "unit": "particle-steps/s", "cores": threads_all, "kind": "port",
         "sample": f"clustered 4,194,304 particles, k=64 (the headline workload): {c3_steps} "
@@ -201,6 +250,10 @@ def cpu_baseline(threads_all: int, c3_steps: int = 1):
         **cpu_info(),
         "one_thread": {"value": v_one, "cores": 1, "seconds_per_step": round(s_one, 3),
                        "sample": "clustered layout at N=32768, k=64, 4 steps after step 0"},
+        "speed_test": {"fps": round(st_fps, 3), "cores": 1, "steps": nst,
+                       "sample": "the reference's speed-test harness input (100,000 uniform "
+                                 "particles, MakeConfig defaults, DeltaTHalf 0.02, g = (0, 0.2)): "
+                                 f"its first {nst} Step() calls, 1 thread"},
     }
 
 
@@ -385,6 +438,8 @@ def main():
     ap.add_argument("--batch-count", type=int, default=0)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
+    ap.add_argument("--no-speed-test", action="store_true",
+                    help="skip the reference's speed-test harness workload")
     ap.add_argument("--errors", action="store_true",
                     help="print the FP32C error report (FP32C vs FP64) and exit")
     ap.add_argument("--jobs", type=int, default=1000, help="served jobs")
@@ -469,9 +524,15 @@ def main():
         line["speedup_fp32c_over_fp64"] = round(res["fp32c"]["value"] / res["fp64"]["value"], 3)
     if not args.no_batch and args.workload == "clustered":
         line["batch"] = run_batch()
+    if not args.no_speed_test and args.workload == "clustered":
+        line["speed_test"] = run_speed_test()
     if not args.no_cpu:
         threads = min(16, cpu_info()["nproc"])
         line["cpu_baseline"] = cpu_baseline(threads, args.cpu_steps)
+        if "speed_test" in line:
+            cpu_fps = line["cpu_baseline"]["speed_test"]["fps"]
+            line["speed_test"]["cpu_1thread_fps"] = cpu_fps
+            line["speed_test"]["fp64_fps_over_cpu"] = round(line["speed_test"]["fp64"]["fps"] / cpu_fps, 1)
     if not args.no_serve and args.workload == "clustered":
         line["serve"] = run_serve(args.jobs, args.serve_precision, args.serve_capacity)
         print(f"# serve: {line['serve']}", file=sys.stderr, flush=True)

@@ -204,6 +204,7 @@ int sph::alloc_state(sph_sim* h, int64_t np, int nitems) {
       (rc = h->alloc(&pb.walk_miss, (int64_t)pb.nsims * kMissSlots * kMissStride)) ||
       (rc = h->alloc(&pb.red_part, (int64_t)pb.nsims * kRedSplit * kNumRed)) ||
       (rc = h->alloc(&pb.retry, np)) || (rc = h->alloc(&pb.retry_count, 1)) ||
+      (rc = h->alloc(&pb.runs, 4 * (int64_t)kRunQueue)) || (rc = h->alloc(&pb.run_count, 1)) ||
       (rc = h->alloc(&pb.knn_cnt, np)) || (rc = h->alloc(&pb.knn_ann, (np + 63) / 64 * 64 * kAnn)) ||
       (rc = h->alloc(&pb.sort.keys, np)) || (rc = h->alloc(&pb.sort.vals, np)) ||
       (rc = h->alloc(&pb.sort.keys_alt, np)) || (rc = h->alloc(&pb.sort.vals_alt, np)) ||
@@ -332,6 +333,7 @@ static int create_batch(int count, const sph_params* const* ps, const int64_t* n
   if (hipMemsetAsync(pb.stats, 0, kStatSlots * sizeof(KnnStats), h->st) != hipSuccess ||
       hipMemsetAsync(pb.walk_miss, 0, (int64_t)pb.nsims * kMissSlots * kMissStride * 4, h->st) !=
           hipSuccess ||
+      hipMemsetAsync(pb.run_count, 0, sizeof(uint32_t), h->st) != hipSuccess ||
       hipMemsetAsync(pb.err, 0, (1 + pb.nsims) * sizeof(int), h->st) != hipSuccess ||
       hipMemsetAsync(pb.nbr, 0, nbr_entries(np, pb.kmax) * sizeof(uint32_t), h->st) != hipSuccess)
     return bail(fail(SPH_EDEVICE, "memset failed"));

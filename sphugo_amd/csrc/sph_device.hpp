@@ -23,6 +23,13 @@ __host__ __device__ inline int red_parts(int nitems) {
 constexpr int kMissSlots = 16;
 constexpr int kMissStride = 32;   // u32 per copy (one cache line)
 constexpr int kAnn = 16;          // kNN annulus entries kept per query
+// cell ranges: a permute wave whose particles span more than kWaveCells
+// cells queues them, in chunks of at most kRunChunk cells, for a parallel
+// fill (kRunQueue chunks per pass; a wave that finds the queue full writes
+// its cells itself)
+constexpr int kWaveCells = 4096;
+constexpr int kRunChunk = 4096;
+constexpr int kRunQueue = 65536;
 
 // Go folds 6*40/(Pi*7), 4*7/(Pi*4), 8*7/(Pi*4) exactly and rounds once
 // (sim/sph.go:266,275,293,303); these are the correctly rounded values.

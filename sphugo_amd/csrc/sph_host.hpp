@@ -67,6 +67,8 @@ struct PassBuffers {
   int red_parts_max;    // the largest red_parts over the simulations (1: one stage)
   uint2* retry;           // kNN retry list (np entries)
   uint32_t* retry_count;
+  uint32_t* runs;          // queued cell chunks of the pass: (first, last, t0, nv) x kRunQueue
+  uint32_t* run_count;
   uint32_t* knn_cnt;       // per query: certain | annulus << 8, or ~0 (retry)
   uint32_t* knn_ann;       // per query: annulus candidates (kAnn entries)
   int have_h;             // every particle has an h from an earlier pass

@@ -504,6 +504,7 @@ extern "C" int sph_server_create(int64_t capacity, int32_t max_jobs, int32_t kma
       hipMemsetAsync(pb.stats, 0, kStatSlots * sizeof(KnnStats), c.st) != hipSuccess ||
       hipMemsetAsync(pb.walk_miss, 0, (int64_t)max_jobs * kMissSlots * kMissStride * 4, c.st) !=
           hipSuccess ||
+      hipMemsetAsync(pb.run_count, 0, sizeof(uint32_t), c.st) != hipSuccess ||
       hipMemsetAsync(pb.err, 0, (1 + max_jobs) * sizeof(int), c.st) != hipSuccess ||
       hipMemsetAsync(pb.nbr, 0, nbr_entries(capacity, pb.kmax) * sizeof(uint32_t), c.st) != hipSuccess ||
       hipStreamSynchronize(c.st) != hipSuccess)

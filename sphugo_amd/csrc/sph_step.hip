@@ -8,6 +8,7 @@
 //                   predictor step (sim/sph.go:108-117) or the step-0
 //                   initialisation (:96-101), write the cell ranges and
 //                   (FP32C) the fixed-point positions
+//   k_fill_runs     the cell ranges of sparse grid stretches k_permute queued
 //   k_knn           tile walk: a wavefront takes 64 consecutive queries,
 //                   stages the candidates of their cells through LDS and
 //                   sorts them against each query's bracket around its last
@@ -189,8 +190,12 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
                                                 const double* __restrict__ VY, int mode,
                                                 uint32_t* __restrict__ keys, int* err,
                                                 uint32_t* __restrict__ retry_count,
+                                                uint32_t* __restrict__ run_count,
                                                 uint32_t hole_cpp) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *retry_count = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    *retry_count = 0u;
+    *run_count = 0u;
+  }
   const Item it = items[xcd_item(nitems)];
   if ((int)threadIdx.x >= it.count) return;
   const int64_t t = it.first + threadIdx.x;
@@ -217,83 +222,142 @@ __global__ __launch_bounds__(kItem) void k_keys(const Item* __restrict__ items, 
   keys[t] = (uint32_t)(S->coff + hilbert2((uint32_t)cx, (uint32_t)cy, S->lg));
 }
 
+// Queues the cells [lo, hi] of a permute wave for k_fill_runs, in chunks of
+// at most kRunChunk cells (wave-uniform call): cell q of a chunk starts at
+// t0 plus the number of the wave's nv keys (skeys[t0 .. t0 + nv)) below q.
+// Returns false, leaving the cells to the caller, when the queue is full
+// (its reserved slots below the queue's end then hold empty chunks).
+__device__ inline bool queue_cells(uint32_t* __restrict__ runs, uint32_t* __restrict__ run_count,
+                                   int64_t lo, int64_t hi, int64_t t0, int nv, int lane) {
+  const uint32_t nch = (uint32_t)((hi - lo + kRunChunk) / kRunChunk);
+  uint32_t at = 0;
+  if (lane == 0) at = atomicAdd(run_count, nch);
+  at = __shfl(at, 0, 64);
+  const bool fits = at + nch <= (uint32_t)kRunQueue;
+  for (uint32_t c = (uint32_t)lane; c < nch && at + c < (uint32_t)kRunQueue; c += 64) {
+    const int64_t f = lo + (int64_t)c * kRunChunk;
+    const int64_t l = f + kRunChunk - 1 < hi ? f + kRunChunk - 1 : hi;
+    uint32_t* r = runs + 4 * (int64_t)(at + c);
+    r[0] = fits ? (uint32_t)f : 1u;
+    r[1] = fits ? (uint32_t)l : 0u;
+    r[2] = (uint32_t)t0;
+    r[3] = (uint32_t)nv;
+  }
+  return fits;
+}
+
+// The queued chunks of cells of this pass, a block per chunk: the chunk's
+// wave keys in LDS, each cell a binary search over them.
+__global__ __launch_bounds__(256) void k_fill_runs(const uint32_t* __restrict__ runs,
+                                                   const uint32_t* __restrict__ run_count,
+                                                   const uint32_t* __restrict__ skeys,
+                                                   uint32_t* __restrict__ cstart) {
+  __shared__ uint32_t keys[64];
+  const uint32_t c = *run_count;
+  const uint32_t n = c < (uint32_t)kRunQueue ? c : (uint32_t)kRunQueue;
+  for (uint32_t r = blockIdx.x; r < n; r += gridDim.x) {  // uniform per block
+    const int64_t f = runs[4 * (int64_t)r], l = runs[4 * (int64_t)r + 1];
+    const uint32_t t0 = runs[4 * (int64_t)r + 2];
+    const int nv = (int)runs[4 * (int64_t)r + 3];
+    __syncthreads();
+    if ((int)threadIdx.x < nv) keys[threadIdx.x] = skeys[(int64_t)t0 + threadIdx.x];
+    __syncthreads();
+    for (int64_t q = f + threadIdx.x; q <= l; q += 256) {
+      int a = 0, b = nv;  // the number of keys below q
+      while (a < b) {
+        const int mid = (a + b) >> 1;
+        if ((int64_t)keys[mid] >= q) b = mid; else a = mid + 1;
+      }
+      cstart[q] = t0 + (uint32_t)a;
+    }
+  }
+}
+
 __global__ __launch_bounds__(kItem) void k_permute(
     const Item* __restrict__ items, int nitems, const SimDev* __restrict__ sims,
     const uint32_t* __restrict__ perm, const uint32_t* __restrict__ skeys, int mode,
     Positions pin, Fields fin, double2* __restrict__ xyp, uint2* __restrict__ uxy, Fields fout,
-    uint8_t* __restrict__ wrap, uint32_t* __restrict__ cstart) {
+    uint8_t* __restrict__ wrap, uint32_t* __restrict__ cstart, uint32_t* __restrict__ runs,
+    uint32_t* __restrict__ run_count) {
   const Item it = items[xcd_item(nitems)];
   if (it.sim < 0) return;  // unused pool slot: stays in place, owns no cells
   const SimDev* S = sims + it.sim;
-  const int lane = threadIdx.x & 63;
-  const int w0 = threadIdx.x & ~63;  // first query of this wave in the item
-  if (w0 >= it.count) return;
   const bool valid = (int)threadIdx.x < it.count;
-  const int64_t t = it.first + (valid ? threadIdx.x : it.count - 1);
+  const int64_t t = it.first + (valid ? threadIdx.x : 0);
   // Cell ranges of this simulation's cells [coff, coff + G^2]: cell q starts
   // at the first particle whose key is >= q; the simulation's last particle
   // closes the range.  The wave writes the contiguous block of cells
   // (key[t0 - 1], key[t_last]] of its particles together, coalesced, each
-  // cell finding its particle by a binary search over the lanes' keys.
-  // Cells outside every simulation's range are never read.
+  // cell finding its particle by a binary search over the lanes' keys; a
+  // block wider than kWaveCells (a sparse stretch of the grid) goes to
+  // k_fill_runs instead, which spreads it over many blocks.  Cells outside
+  // every simulation's range are never read.
   {
-    const int64_t t0 = it.first + w0;
-    const int nv = (it.count - w0) < 64 ? (it.count - w0) : 64;
-    const uint32_t kt = skeys[t];
-    const int64_t lo = (t0 == S->off ? S->coff : (int64_t)skeys[t0 - 1] + 1);
-    const int64_t hi = (int64_t)__shfl(kt, nv - 1, 64);
-    for (int64_t qb = lo; qb <= hi; qb += 64) {  // uniform: every lane shuffles
-      const int64_t q = qb + lane;
-      int a = 0, b = nv - 1;  // smallest lane with key >= q (lane nv-1 qualifies)
-      for (int st = 0; st < 6; ++st) {
-        const int mid = (a + b) >> 1;
-        const int64_t km = (int64_t)__shfl(kt, mid, 64);
-        if (km >= q) b = mid; else a = mid + 1;
+    const int lane = threadIdx.x & 63;
+    const int w0 = threadIdx.x & ~63;
+    if (w0 < it.count) {
+      const int64_t t0 = it.first + w0;
+      const int nv = (it.count - w0) < 64 ? (it.count - w0) : 64;
+      const uint32_t kt = skeys[t0 + (lane < nv ? lane : nv - 1)];
+      const int64_t lo = (t0 == S->off ? S->coff : (int64_t)skeys[t0 - 1] + 1);
+      const int64_t hi = (int64_t)__shfl(kt, nv - 1, 64);
+      if (hi - lo + 1 <= kWaveCells || !queue_cells(runs, run_count, lo, hi, t0, nv, lane)) {
+        for (int64_t qb = lo; qb <= hi; qb += 64) {  // uniform: every lane shuffles
+          const int64_t q = qb + lane;
+          int a = 0, b = nv - 1;  // smallest lane with key >= q (lane nv-1 qualifies)
+          for (int st = 0; st < 6; ++st) {
+            const int mid = (a + b) >> 1;
+            const int64_t km = (int64_t)__shfl(kt, mid, 64);
+            if (km >= q) b = mid; else a = mid + 1;
+          }
+          if (q <= hi) cstart[q] = (uint32_t)(t0 + a);
+        }
       }
-      if (q <= hi) cstart[q] = (uint32_t)(t0 + a);
-    }
-    const int64_t tl = t0 + nv - 1;
-    if (tl == S->off + S->n - 1) {
-      const int64_t end = S->coff + (int64_t)S->gx * S->gy;
-      for (int64_t q = hi + 1 + lane; q <= end; q += 64) cstart[q] = (uint32_t)(tl + 1);
+      const int64_t tl = t0 + nv - 1;
+      if (tl == S->off + S->n - 1) {
+        const int64_t end = S->coff + (int64_t)S->gx * S->gy;
+        if (end - hi <= kWaveCells || !queue_cells(runs, run_count, hi + 1, end, tl + 1, 0, lane))
+          for (int64_t q = hi + 1 + lane; q <= end; q += 64) cstart[q] = (uint32_t)(tl + 1);
+      }
     }
   }
-  if (!valid) return;
-  const uint32_t i = perm[t];
-  if (mode < 0) mode = S->pmode;
-  double x = pin.x[i], y = pin.y[i];
-  const double vx = fin.vx[i], vy = fin.vy[i], e = fin.e[i];
-  double vpx, vpy, ep;
-  if (mode == kPermPredict) {
-    const double dt = S->dt_half;
-    x = x + vx * dt;
-    y = y + vy * dt;
-    const double adx = fin.ax[i] * dt, ady = fin.ay[i] * dt;
-    vpx = vx + adx;
-    vpy = vy + ady;
-    ep = e + fin.edot[i] * dt;
-  } else if (mode == kPermInit) {
-    vpx = vx; vpy = vy; ep = e;
-  } else {
-    vpx = fin.vpx[i]; vpy = fin.vpy[i]; ep = fin.ep[i];
+  if (valid) {
+    const uint32_t i = perm[t];
+    if (mode < 0) mode = S->pmode;
+    double x = pin.x[i], y = pin.y[i];
+    const double vx = fin.vx[i], vy = fin.vy[i], e = fin.e[i];
+    double vpx, vpy, ep;
+    if (mode == kPermPredict) {
+      const double dt = S->dt_half;
+      x = x + vx * dt;
+      y = y + vy * dt;
+      const double adx = fin.ax[i] * dt, ady = fin.ay[i] * dt;
+      vpx = vx + adx;
+      vpy = vy + ady;
+      ep = e + fin.edot[i] * dt;
+    } else if (mode == kPermInit) {
+      vpx = vx; vpy = vy; ep = e;
+    } else {
+      vpx = fin.vpx[i]; vpy = fin.vpy[i]; ep = fin.ep[i];
+    }
+    xyp[t] = make_double2(x, y);
+    // FP32C: the fixed-point form of the force record (units of L / 2^32,
+    // wrapped), whose 32-bit differences are minimum-image displacements
+    if (S->prec == kPrecFP32C)
+      uxy[t] = make_uint2((uint32_t)(int64_t)floor((x - S->x0) * (4294967296.0 / S->lx)),
+                          (uint32_t)(int64_t)floor((y - S->y0) * (4294967296.0 / S->ly)));
+    fout.vx[t] = vx;
+    fout.vy[t] = vy;
+    fout.e[t] = e;
+    fout.vpx[t] = vpx;
+    fout.vpy[t] = vpy;
+    fout.ep[t] = ep;
+    fout.h[t] = fin.h[i];
+    fout.id[t] = fin.id[i];
+    CellIdx c = cell_of(S, x, y);
+    const int64_t wx = floordiv_pow2(c.ux, S->lg), wy = floordiv_pow2(c.uy, S->lg);
+    wrap[t] = (uint8_t)((wx + 2) | ((wy + 2) << 3));
   }
-  xyp[t] = make_double2(x, y);
-  // FP32C: the fixed-point form of the force record (units of L / 2^32,
-  // wrapped), whose 32-bit differences are minimum-image displacements
-  if (S->prec == kPrecFP32C)
-    uxy[t] = make_uint2((uint32_t)(int64_t)floor((x - S->x0) * (4294967296.0 / S->lx)),
-                        (uint32_t)(int64_t)floor((y - S->y0) * (4294967296.0 / S->ly)));
-  fout.vx[t] = vx;
-  fout.vy[t] = vy;
-  fout.e[t] = e;
-  fout.vpx[t] = vpx;
-  fout.vpy[t] = vpy;
-  fout.ep[t] = ep;
-  fout.h[t] = fin.h[i];
-  fout.id[t] = fin.id[i];
-  CellIdx c = cell_of(S, x, y);
-  const int64_t wx = floordiv_pow2(c.ux, S->lg), wy = floordiv_pow2(c.uy, S->lg);
-  wrap[t] = (uint8_t)((wx + 2) | ((wy + 2) << 3));
 }
 
 // ------------------------------------------------------------------------
@@ -1950,7 +2014,7 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   }
   hipLaunchKernelGGL(k_keys, grd_all, blk, 0, st, b.items_all, b.nitems_all, b.sims, b.pos_cur.x,
                      b.pos_cur.y, b.fcur.vx, b.fcur.vy, perm_mode, b.sort.keys, b.err,
-                     b.retry_count, b.hole_cpp);
+                     b.retry_count, b.run_count, b.hole_cpp);
   mark(0, 1);
   mark(1, 0);
   uint32_t *skeys = nullptr, *perm = nullptr;
@@ -1958,7 +2022,10 @@ int enqueue_pass(PassBuffers& b, int perm_mode, int force_mode, hipStream_t st,
   mark(1, 1);
   mark(2, 0);
   hipLaunchKernelGGL(k_permute, grd_all, blk, 0, st, b.items_all, b.nitems_all, b.sims, perm, skeys,
-                     perm_mode, b.pos_cur, b.fcur, b.xyp, b.uxy, b.fnew, b.wrap, b.cstart);
+                     perm_mode, b.pos_cur, b.fcur, b.xyp, b.uxy, b.fnew, b.wrap, b.cstart, b.runs,
+                     b.run_count);
+  hipLaunchKernelGGL(k_fill_runs, dim3(1024), dim3(256), 0, st, b.runs, b.run_count, skeys,
+                     b.cstart);
   mark(2, 1);
   mark(3, 0);
   {

@@ -20,6 +20,8 @@ Workloads (N, k, layout):
               sigma .02) + 20% uniform, v ~ N(0, .1^2) per axis, e=1 (seed 3)
   batch       512 simulations, N_i = floor(exp(U(ln 1024, ln 65536))),
               each a `blast` layout with seed 4*10**6 + i, k=32
+  speed_test  N=100,000, k=32: the reference's timing harness input, uniform
+              in the unit square, v=0, e=0.01, mass 1 (seed 12345678)
 """
 from __future__ import annotations
 
@@ -133,6 +135,20 @@ def clustered(n: int = 4194304, seed: int = 3, k: int = 64, nclusters: int = 16,
     vx = vsigma * s.normal(n)
     vy = vsigma * s.normal(n)
     return _state(x, y, vx=vx, vy=vy, k=k, seed=seed, name=f"clustered-{n}")
+
+
+def speed_test(n: int = 100000, seed: int = 12345678):
+    """The input of the reference's timing harness
+    (examples/speed-test/speed-test.go:22-28): a UniformRectSpawner of n
+    particles over the unit square (sim/config-parser.go:50-80: at rest,
+    E = 0.01), with MakeConfig's ParticleMass 1 (sim/config-parser.go:131-149).
+    Positions come from this module's generator, not Go's math/rand."""
+    s = Stream(seed)
+    x = s.uniform(n)
+    y = s.uniform(n)
+    st = _state(x, y, e=np.full(n, 0.01), k=32, seed=seed, name=f"speed-test-{n}")
+    st["mass"] = 1.0
+    return st
 
 
 def batch_sizes(count: int = 512, seed: int = 4, lo: int = 1024, hi: int = 65536):
