@@ -2,8 +2,9 @@
 reflection walls, the Wendland kernel over several steps, anisotropic and
 shifted boxes, exact distance ties on lattices, the smallest k and N, a
 batch whose simulations have different boxes, FP32C neighbourhoods wider
-than half a box, and a pair of particles closer than the FP32C force
-record's fixed-point step."""
+than half a box, a pair of particles closer than the FP32C force record's
+fixed-point step, and a grid mostly empty between two patches (its cell
+ranges over long empty stretches filled by the queued path)."""
 import numpy as np
 import pytest
 
@@ -214,3 +215,35 @@ def test_batch_with_different_boxes():
         o.step(3)
         for f in ("x", "y", "vx", "vy", "e", "rho", "h"):
             assert (b.get_sim(i, f) == o.get(f)).all(), (i, f)
+
+
+def _two_patches(n, seed):
+    """n particles in two 0.06-wide patches at opposite corners of the unit
+    box: with 4 cells per particle the Hilbert curve crosses stretches of
+    ~10^5 empty cells before, between and after them."""
+    rng = np.random.default_rng(seed)
+    h = n // 2
+    x = np.concatenate([0.10 + 0.06 * rng.random(h), 0.82 + 0.06 * rng.random(n - h)])
+    y = np.concatenate([0.12 + 0.06 * rng.random(h), 0.80 + 0.06 * rng.random(n - h)])
+    return {"x": x, "y": y, "vx": 0.05 * rng.standard_normal(n),
+            "vy": 0.05 * rng.standard_normal(n), "e": np.ones(n), "mass": 1.0 / n}
+
+
+def test_sparse_grid_cell_ranges():
+    """Cell ranges over long empty stretches (the queued fill): bitwise steps
+    equal the oracle bit for bit, and after plain FP64 steps a walk pass finds
+    the oracle's neighbour sets and forces."""
+    st = _two_patches(20000, seed=41)
+    g, o = _pair(st, 32, flags=L.FLAG_BITWISE, forces=False, dt_half=0.0, cfl=0.3)
+    g.step(3)
+    o.step(3)
+    for f in ("x", "y", "vx", "vy", "e", "rho", "h"):
+        assert (g.get(f) == o.get(f)).all(), f
+    from test_gpu_parity import state_pass_vs_oracle
+    common = dict(k=32, mass=st["mass"], gamma=5 / 3, hor=(0, 1), vert=(0, 1), dt_half=0.0,
+                  cfl=0.3, visc_alpha=1.0, visc_beta=2.0)
+    p = L.Sim(L.default_params(**common), st["x"], st["y"], st["vx"], st["vy"], st["e"])
+    p.step(3)
+    o2, _ = state_pass_vs_oracle(p, O.default_params(knn_mode=O.KNN_EXACT, use_cfl=1, **common))
+    _check_knn(p, o2)
+    _check_forces(p, o2)
