@@ -440,6 +440,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-speed-test", action="store_true",
                     help="skip the reference's speed-test harness workload")
+    ap.add_argument("--speed-test-only", action="store_true",
+                    help="run only the reference's speed-test harness workload (GPU) and print it")
     ap.add_argument("--errors", action="store_true",
                     help="print the FP32C error report (FP32C vs FP64) and exit")
     ap.add_argument("--jobs", type=int, default=1000, help="served jobs")
@@ -464,6 +466,9 @@ def main():
         sys.exit(2)
     if args.errors:
         print(json.dumps({"fp32c_error_report": error_report()}), flush=True)
+        return
+    if args.speed_test_only:
+        print(json.dumps({"speed_test": run_speed_test()}), flush=True)
         return
     if args.serve_only:
         print(json.dumps({"serve": run_serve(args.jobs, args.serve_precision,
