@@ -103,7 +103,7 @@ typedef struct sph_params {
   double refl[4];     /* Reflections L, R, U, D */
   double visc_alpha;  /* 0.75 in the reference */
   double visc_beta;   /* 1.5 */
-  double visc_eta2;   /* 0.01 */
+  double visc_eta2;   /* 0.01; must be > 0 */
 } sph_params;
 
 typedef struct sph_sim sph_sim;

@@ -78,6 +78,11 @@ int sph::check_params(const sph_params* p, int64_t n) {
     return fail(SPH_EINVAL, "fewer than k candidates over the %lld images", (long long)images);
   if (!(p->gamma > 1) && !(p->gamma < 1)) return fail(SPH_EINVAL, "gamma must not be 1");
   if (p->dt_half <= 0 && !(p->cfl > 0)) return fail(SPH_EINVAL, "CFL number must be positive");
+  // the viscosity's softening (the reference's constant 0.01, sim/sph.go:379-383)
+  // keeps mu finite as two approaching particles meet; the FP32C force
+  // evaluates it for every pair, so it must be a positive finite number
+  if (!(p->visc_eta2 > 0) || !std::isfinite(p->visc_eta2))
+    return fail(SPH_EINVAL, "visc_eta2 must be positive and finite, got %g", p->visc_eta2);
   return 0;
 }
 

@@ -55,6 +55,8 @@ def _create(**kw):
     (dict(n=3, k=32), "fewer than k"),
     (dict(gamma=1.0), "gamma"),
     (dict(dt_half=0.0, cfl=0.0), "CFL"),
+    (dict(visc_eta2=0.0), "visc_eta2"),
+    (dict(visc_eta2=-0.01), "visc_eta2"),
 ])
 def test_create_rejects(kw, needle):
     rc, msg = _create(**kw)
